@@ -1,0 +1,302 @@
+/*
+ * tressfx_amd.h — C-ABI of the MI355X-native TressFX 4.1 strand simulation and
+ * SDF collision path.
+ *
+ * Every entry point returns an int status (TFX_OK == 0); on failure a
+ * thread-local message is available from tfx_last_error().  No entry point
+ * aborts.  Device pointers are plain `void*`/`float*` into HBM; `stream` is a
+ * hipStream_t passed as `void*` (NULL = the default stream).  Calls on one
+ * object are stream-ordered and not internally synchronised: use one stream per
+ * object at a time.
+ *
+ * Data layout at the boundary is the reference's (renderer contract):
+ *   - per-vertex streams are float4 AoS, strand-major (index = strand*V + v);
+ *     follow strands are interleaved after their guide (strand = g*(F+1)+j);
+ *   - w of a position is its inverse mass (0 = pinned);
+ *   - bone matrices are row-major, row-vector convention (p' = [p,1]*M,
+ *     translation in row 3), 16 floats each.
+ * Citations are to files of the TressFX 4.1 reference tree.
+ */
+#ifndef TRESSFX_AMD_H_
+#define TRESSFX_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFX_SIM_THREAD_GROUP_SIZE 64       /* src/TressFX/TressFXCommon.h:34 */
+#define TFX_MAX_NUM_BONES 512              /* src/TressFX/AMD_TressFX.h:45-47 */
+#define TFX_MAX_NUM_COLLISION_CAPSULES 8   /* src/TressFX/TressFXCommon.h:32 */
+#define TFX_MIN_VERTS_PER_STRAND_FOR_GPU_ITERATION 64 /* src/TressFX/TressFXHairObject.cpp:40 */
+
+enum tfx_status {
+    TFX_OK = 0,
+    TFX_ERR_INVALID_ARGUMENT = 1,
+    TFX_ERR_OUT_OF_MEMORY = 2,
+    TFX_ERR_HIP = 3,
+    TFX_ERR_ADMISSION = 4,
+    TFX_ERR_NOT_FOUND = 5,
+    TFX_ERR_UNSUPPORTED = 6
+};
+
+/* ------------------------------------------------------------------------ */
+/* Records and parameter blocks                                              */
+/* ------------------------------------------------------------------------ */
+
+/* TressFXSimulationSettings, src/TressFX/TressFXSettings.h:35-87.
+ * tfx_simulation_settings_default() fills the reference defaults (:38-56). */
+typedef struct tfx_simulation_settings {
+    float vsp_coeff;                      /* m_vspCoeff */
+    float vsp_accel_threshold;            /* m_vspAccelThreshold */
+    float local_constraint_stiffness;     /* m_localConstraintStiffness */
+    int   local_constraints_iterations;   /* m_localConstraintsIterations */
+    float global_constraint_stiffness;    /* m_globalConstraintStiffness */
+    float global_constraints_range;       /* m_globalConstraintsRange */
+    int   length_constraints_iterations;  /* m_lengthConstraintsIterations */
+    float damping;                        /* m_damping */
+    float gravity_magnitude;              /* m_gravityMagnitude */
+    float tip_separation;                 /* m_tipSeparation */
+    float wind_magnitude;                 /* m_windMagnitude */
+    float wind_direction[3];              /* m_windDirection */
+    float wind_angle_radians;             /* m_windAngleRadians (unused by the path) */
+    float clamp_position_delta;           /* m_clampPositionDelta (ignored: 20 is hard-coded, TressFXHairObject.cpp:397) */
+} tfx_simulation_settings;
+
+/* TressFXSimulationParams, src/TressFX/TressFXConstantBuffers.h:41-110, i.e. the
+ * HLSL cbuffer tressfxSimParameters (src/Shaders/TressFXSimulation.hlsl:49-79),
+ * plus the capsule block of the TRESSFX_COLLISION_CAPSULES build (:72-76) and
+ * the per-object host state the dispatch sequence reads
+ * (m_CPULocalShapeIterations, TressFXHairObject.h:120).  Bone matrices are kept
+ * apart (tfx_hair_update_bone_matrices). */
+typedef struct tfx_simulation_params {
+    float wind[4];            /* g_Wind  (w unused) */
+    float wind1[4];           /* g_Wind1 */
+    float wind2[4];           /* g_Wind2 */
+    float wind3[4];           /* g_Wind3 */
+    float shape[4];           /* damping, local stiffness, global stiffness, global range */
+    float grav_time_tip[4];   /* gravity magnitude, time step, tip separation, unused */
+    int   sim_ints[4];        /* length iterations, local iterations, collision flag, unused */
+    int   counts[4];          /* strands per 64-thread group, follow per guide, vertices per strand, unused */
+    float vsp[4];             /* vsp coefficient, vsp accel threshold, unused, unused */
+    float reset_positions;    /* g_ResetPositions */
+    float clamp_position_delta; /* g_ClampPositionDelta */
+    float pad1;
+    float pad2;
+    float center_and_radius0[TFX_MAX_NUM_COLLISION_CAPSULES][4]; /* g_centerAndRadius0 */
+    float center_and_radius1[TFX_MAX_NUM_COLLISION_CAPSULES][4]; /* g_centerAndRadius1 */
+    int   num_collision_capsules[4];                             /* g_numCollisionCapsules.x */
+    int   cpu_local_shape_iterations; /* LSC re-dispatch count, TressFXHairObject.cpp:402-411 */
+    int   num_bones;                  /* matrices supplied by UpdateBoneMatrices */
+    int   simulation_frame;           /* m_SimulationFrame at packing time */
+    int   pad3;
+} tfx_simulation_params;
+
+/* TressFXBoneSkinningData, src/TressFX/TressFXAsset.h:42-46 (indices stored as float). */
+typedef struct tfx_bone_skinning_data {
+    float bone_index[4];
+    float bone_weight[4];
+} tfx_bone_skinning_data;
+
+/* TressFXStrandLevelData, src/TressFX/TressFXHairObject.h:78-83. */
+typedef struct tfx_strand_level_data {
+    float skinning_quat[4];
+    float vsp_quat[4];
+    float vsp_translation[4]; /* xyz translation, w = VSP coefficient */
+} tfx_strand_level_data;
+
+/* StandardVertex, src/Shaders/TressFXSDFCollision.hlsl:35-39. */
+typedef struct tfx_standard_vertex {
+    float position[4];
+    float normal[4];
+} tfx_standard_vertex;
+
+/* The host arrays of a processed TressFXAsset (src/TressFX/TressFXAsset.h:50-100):
+ * what the TressFXHairObject constructor copies to the GPU
+ * (TressFXHairObject.cpp:44-164).  All arrays are copied; the caller may free
+ * them after tfx_hair_create returns. */
+typedef struct tfx_asset_desc {
+    int num_guide_strands;              /* m_numGuideStrands */
+    int num_follow_strands_per_guide;   /* m_numFollowStrandsPerGuide */
+    int num_vertices_per_strand;        /* m_numVerticesPerStrand: 4, 8, 16, 32 or 64 */
+    int pad;
+    const float* positions;             /* float4[S*V]  m_positions (also the rest pose) */
+    const float* tangents;              /* float4[S*V]  m_tangents (may be NULL: zeros) */
+    const float* rest_lengths;          /* float [S*V]  m_restLengths */
+    const float* follow_root_offsets;   /* float4[S]    m_followRootOffsets (may be NULL if F == 0) */
+    const tfx_bone_skinning_data* bone_skinning_data; /* [S] m_boneSkinningData */
+} tfx_asset_desc;
+
+typedef struct tfx_hair_info {
+    int num_guide_strands;
+    int num_follow_strands_per_guide;
+    int num_vertices_per_strand;
+    int num_total_strands;     /* GetNumTotalHairStrands */
+    int num_total_vertices;    /* GetNumTotalHairVertices */
+    int simulation_frame;      /* m_SimulationFrame */
+    int cpu_local_shape_iterations; /* GetCPULocalShapeIterations */
+    int pad;
+} tfx_hair_info;
+
+/* TressFXDynamicState (TressFXHairObject.h:54-76): device pointers. */
+typedef struct tfx_dynamic_state {
+    float* positions;           /* float4[N] */
+    float* positions_prev;      /* float4[N] */
+    float* positions_prev_prev; /* float4[N] */
+    float* tangents;            /* float4[N] */
+    tfx_strand_level_data* strand_level_data; /* [S] */
+} tfx_dynamic_state;
+
+/* What TressFXSDFInputMeshInterface (src/TressFX/TressFXSDFInputMeshInterface.h:33-57)
+ * hands the SDF builder.  vertices/indices are borrowed device pointers that
+ * must stay valid through every tfx_sdf_update that uses them. */
+typedef struct tfx_sdf_mesh_desc {
+    const tfx_standard_vertex* vertices;  /* GetMeshBuffer(), device */
+    const uint32_t* indices;              /* GetTrimeshVertexIndicesBuffer(), device, 3 per triangle */
+    int num_vertices;                     /* GetNumMeshVertices() */
+    int num_triangles;                    /* GetNumMeshTriangle() */
+    float bbox_min[3];                    /* GetBoundingBox() */
+    float bbox_max[3];
+    float initial_bbox_min[3];            /* GetInitialBoundingBox() */
+    float initial_bbox_max[3];
+} tfx_sdf_mesh_desc;
+
+/* TressFXSDFCollisionParams, src/TressFX/TressFXConstantBuffers.h:119-134. */
+typedef struct tfx_sdf_collision_params {
+    float origin[4];
+    float cell_size;
+    int   num_cells_x;
+    int   num_cells_y;
+    int   num_cells_z;
+    int   max_marching_cubes_vertices;
+    float marching_cubes_isolevel;
+    float collision_margin;          /* already scaled by the cell size */
+    int   num_hair_vertices_per_strand;
+    int   num_total_hair_vertices;
+    float pad1, pad2, pad3;
+} tfx_sdf_collision_params;
+
+typedef struct tfx_sdf_grid_info {
+    float origin[3];
+    float cell_size;
+    int   num_cells_x, num_cells_y, num_cells_z;
+    int   num_total_cells;     /* allocation: 1.4 * X*Y*Z for the reference constructor */
+    float collision_margin;    /* in cell units, as passed to the constructor */
+    float padding_boundary;
+    int   explicit_grid;
+    int   pad;
+} tfx_sdf_grid_info;
+
+typedef struct tfx_hair tfx_hair;
+typedef struct tfx_simulation tfx_simulation;
+typedef struct tfx_sdf_collision_system tfx_sdf_collision_system;
+typedef struct tfx_sdf tfx_sdf;
+typedef struct tfx_batch tfx_batch;
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+const char* tfx_last_error(void);
+int tfx_get_version(int* major, int* minor, int* patch); /* AMD_TressFX.h:33-35 */
+/* Selects the HIP device used by objects created afterwards on this thread. */
+int tfx_init(int hip_device);
+void tfx_simulation_settings_default(tfx_simulation_settings* out);
+
+/* ------------------------------------------------------------------------ */
+/* TressFXHairObject  (src/TressFX/TressFXHairObject.h:85-191)               */
+/* ------------------------------------------------------------------------ */
+/* TressFXHairObject(TressFXAsset*, ...) — TressFXHairObject.cpp:44-164.
+ * Pos = Prev = PrevPrev = positions (TressFXDynamicState::UploadGPUData, :528-558),
+ * then ResetPositions() (:157). Synchronises the device before returning. */
+int tfx_hair_create(const tfx_asset_desc* asset, tfx_hair** out);
+int tfx_hair_destroy(tfx_hair* hair);
+/* UpdateBoneMatrices, TressFXHairObject.cpp:366-373 (count clamped to 512). */
+int tfx_hair_update_bone_matrices(tfx_hair* hair, const float* bone_matrices_ws, int num_bone_matrices);
+/* UpdateSimulationParameters, TressFXHairObject.cpp:380-437 (+ SetWind :191-235). */
+int tfx_hair_update_simulation_parameters(tfx_hair* hair, const tfx_simulation_settings* settings, float time_step);
+/* The capsule block of the TRESSFX_COLLISION_CAPSULES build (TressFXHairObject.cpp:422-431):
+ * center_and_radius0/1 are float4[n] (xyz centre, w radius); n <= 8. */
+int tfx_hair_set_collision_capsules(tfx_hair* hair, const float* center_and_radius0,
+                                    const float* center_and_radius1, int num_capsules);
+/* ResetPositions, TressFXHairObject.h:104. */
+int tfx_hair_reset_positions(tfx_hair* hair);
+/* IncreaseSimulationFrame, TressFXHairObject.h:129 (tfx_simulation_simulate calls it). */
+int tfx_hair_increase_simulation_frame(tfx_hair* hair);
+/* GetNumTotalHairVertices / Strands / VerticesPerStrand / FollowHairsPerGuideHair (:114-118). */
+int tfx_hair_get_info(const tfx_hair* hair, tfx_hair_info* out);
+/* The parameter block the next Simulate will use (m_SimCB[m_SimulationFrame % 2]). */
+int tfx_hair_get_simulation_params(const tfx_hair* hair, tfx_simulation_params* out,
+                                   float* bone_matrices_out /* 16*num_bones or NULL */);
+/* GetDynamicState, TressFXHairObject.h:112. */
+int tfx_hair_get_dynamic_state(tfx_hair* hair, tfx_dynamic_state* out);
+/* Snapshot / restore of the dynamic state (host arrays; NULL entries skipped).
+ * Both are stream-ordered and synchronise `stream` before returning. */
+int tfx_hair_download_state(tfx_hair* hair, void* stream, float* positions, float* positions_prev,
+                            float* positions_prev_prev, float* tangents, float* strand_level_data);
+int tfx_hair_upload_state(tfx_hair* hair, void* stream, const float* positions, const float* positions_prev,
+                          const float* positions_prev_prev, const float* tangents,
+                          const float* strand_level_data);
+
+/* ------------------------------------------------------------------------ */
+/* TressFXSimulation  (src/TressFX/TressFXSimulation.h:34-48)                */
+/* ------------------------------------------------------------------------ */
+/* Initialize, TressFXSimulation.cpp:34-43. */
+int tfx_simulation_create(tfx_simulation** out);
+int tfx_simulation_destroy(tfx_simulation* sim);
+/* Simulate, TressFXSimulation.cpp:77-112: the reference's dispatch structure —
+ * for each of the six stages, one launch per hair object (LocalShapeConstraints
+ * repeated GetCPULocalShapeIterations() times), then IncreaseSimulationFrame. */
+int tfx_simulation_simulate(tfx_simulation* sim, void* stream, tfx_hair* const* hairs, int num_hairs);
+/* Runs a single stage (0 Integration+GlobalShape, 1 StrandLevelData, 2 VSP,
+ * 3 LocalShapeConstraints (one pass), 4 LengthConstraintsWindAndCollision,
+ * 5 UpdateFollowHairVertices) on one hair with its current parameters; for
+ * stage-by-stage parity checks.  Does not advance the frame. */
+int tfx_simulation_run_stage(tfx_simulation* sim, void* stream, tfx_hair* hair, int stage);
+/* One substep of all six stages as a single fused kernel per hair (strand
+ * blocks staged in LDS), followed by the hair's SDF collision if `sdf` is
+ * non-NULL; then IncreaseSimulationFrame.  Same results as
+ * tfx_simulation_simulate + tfx_sdf_collide_with_hair, except that PrevPrev is
+ * maintained only for vertices 0 and 1 and StrandLevelData is not written. */
+int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, tfx_hair* const* hairs,
+                                  tfx_sdf* const* sdfs, int num_hairs);
+/* Launch shape of the fused kernel for strands of `vertices_per_strand`
+ * vertices: `variant` 0 = 1024-vertex strand blocks, 1 = 2048-vertex blocks;
+ * `blocks_per_cu` sizes the persistent grid (0 keeps the current value).
+ * Results do not depend on the shape. */
+int tfx_simulation_set_launch_shape(tfx_simulation* sim, int vertices_per_strand, int variant, int blocks_per_cu);
+
+/* ------------------------------------------------------------------------ */
+/* TressFXSDFCollisionSystem / TressFXSDFCollision                          */
+/* (src/TressFX/TressFXSDFCollision.h:40-124)                               */
+/* ------------------------------------------------------------------------ */
+int tfx_sdf_collision_system_create(tfx_sdf_collision_system** out); /* Initialize, .h:43-50 */
+int tfx_sdf_collision_system_destroy(tfx_sdf_collision_system* sys);
+/* TressFXSDFCollision(pDevice, pCollMesh, name, numCellsInX, collisionMargin),
+ * TressFXSDFCollision.cpp:38-86: grid sized from the mesh's initial bounding box. */
+int tfx_sdf_create(const tfx_sdf_mesh_desc* mesh, int num_cells_in_x, float collision_margin, tfx_sdf** out);
+/* Explicit grid (origin fixed, cell size and dimensions given). */
+int tfx_sdf_create_explicit(const tfx_sdf_mesh_desc* mesh, const float origin[3], float cell_size,
+                            int num_cells_x, int num_cells_y, int num_cells_z,
+                            float collision_margin, tfx_sdf** out);
+int tfx_sdf_destroy(tfx_sdf* sdf);
+/* Replace the mesh source (new device buffers and/or bounding box). */
+int tfx_sdf_set_mesh(tfx_sdf* sdf, const tfx_sdf_mesh_desc* mesh);
+/* Update, TressFXSDFCollision.cpp:95-169: origin from the current bbox, then
+ * Initialize -> Construct -> Finalize. */
+int tfx_sdf_update(tfx_sdf* sdf, tfx_sdf_collision_system* sys, void* stream);
+/* CollideWithHair, TressFXSDFCollision.cpp:171-221. */
+int tfx_sdf_collide_with_hair(tfx_sdf* sdf, tfx_sdf_collision_system* sys, void* stream, tfx_hair* hair);
+int tfx_sdf_get_grid(const tfx_sdf* sdf, tfx_sdf_grid_info* out);  /* grid getters, .h:76-93 */
+/* GetSDFDataGPUBuffer, .h:71-72: device float[X*Y*Z] (bits as written by Finalize). */
+int tfx_sdf_get_data_buffer(tfx_sdf* sdf, float** device_ptr);
+int tfx_sdf_download(tfx_sdf* sdf, void* stream, float* host_cells /* X*Y*Z */);
+/* The constant block CollideWithHair would upload for `hair` (TressFXSDFCollision.cpp:184-198). */
+int tfx_sdf_get_collision_params(const tfx_sdf* sdf, const tfx_hair* hair, tfx_sdf_collision_params* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRESSFX_AMD_H_ */

@@ -1,0 +1,272 @@
+"""GPU parity of the HIP path against the CPU restatement (oracle).
+
+From identical state, every kernel must reproduce the FP32 transcription bit
+for bit (both evaluate the reference's expressions in the same order with IEEE
+division/sqrt and no contraction), and every substep must be within 1e-5
+relative per vertex position of the FP64 oracle.  Multi-frame runs are
+compared stepwise: before every substep the GPU state is loaded into the
+oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tressfx_amd as tfx
+from tressfx_amd import synth
+from tfx_testutil import (REL_TOL, DeviceMesh, bits_equal, first_mismatch, oracle_pair, rel_err,
+                          set_substep_inputs)
+
+pytestmark = pytest.mark.gpu
+
+STAGE_NAMES = ["IntegrationAndGlobalShapeConstraints", "CalculateStrandLevelData", "VelocityShockPropagation",
+               "LocalShapeConstraints", "LengthConstriantsWindAndCollision", "UpdateFollowHairVertices"]
+
+
+def small_workload(G, V, F, wind=True, capsules=True, seed=7):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    roots, nrm = synth.hemisphere_roots(G, 0.1, rng)
+    strands = synth.grow_strands(roots, nrm, V, 0.005, 0.15, rng)
+    asset = synth.build_asset(strands, F, rng, follow_sigma=0.002, tip_separation=0.5 if F else 0.0)
+    s = synth.settings_config2()
+    s.tip_separation = 0.5
+    if not wind:
+        s.wind_magnitude = 0.0
+    caps = None
+    if capsules:
+        caps = (np.array([[0.0, 0.0, 0.0, 0.1], [0.02, 0.1, 0.0, 0.03]], np.float32),
+                np.array([[0.0, 1e-3, 0.0, 0.1], [0.05, 0.14, 0.0, 0.02]], np.float32))
+    wl = synth.Workload(asset, s, 2, lambda t: synth.yaw_pitch_matrix(0.6 * np.sin(2 * np.pi * t),
+                                                                       0.3 * np.sin(2 * np.pi * 0.7 * t)),
+                        capsules=caps, wind_fn=(lambda t: synth.wind_gust(t)) if wind else None)
+    return wl
+
+
+def make_hair(wl):
+    hair = tfx.TressFXHairObject(wl.asset)
+    if wl.capsules is not None:
+        hair.SetCollisionCapsules(*wl.capsules)
+    return hair
+
+
+def compare_state_fp32(gpu_st, orc, what, keys=("pos", "prev", "prevprev", "tangent")):
+    for k in keys:
+        mm = first_mismatch(gpu_st[k], getattr(orc, k))
+        assert mm is None, f"{what}: {k} differs from the FP32 oracle at {mm[0]} gpu={mm[1]!r} oracle={mm[2]!r} ({mm[3]} words)"
+
+
+@pytest.mark.parametrize("V,F", [(16, 0), (32, 2), (4, 1), (8, 0), (64, 1)])
+def test_stages_bit_exact_vs_fp32_oracle(gpu, V, F):
+    """Each of the six kernels, one at a time, from the GPU's own state."""
+    print(f"[stages] V={V} F={F}", flush=True)
+    wl = small_workload(96, V, F)
+    hair = make_hair(wl)
+    sim = tfx.TressFXSimulation()
+    o32 = oracle.OracleHair(wl.asset, np.float32)
+    for frame in range(3):
+        for sub in range(wl.substeps):
+            set_substep_inputs(hair, wl, frame, sub)
+            params, bones = hair.simulation_params()
+            o32.set_state(hair.download_state())
+            for stage in range(6):
+                reps = params.cpu_local_shape_iterations if stage == 3 else 1
+                for _ in range(reps):
+                    sim.RunStage(hair, stage)
+                    o32.run_stage(params, bones, stage)
+                st = hair.download_state()
+                compare_state_fp32(st, o32, f"frame {frame} sub {sub} stage {STAGE_NAMES[stage]}")
+                if stage in (1,):
+                    G = wl.asset.num_guide_strands
+                    gi = np.arange(G) * (F + 1)
+                    mm = first_mismatch(st["strand_level"][gi], o32.strand_level[gi])
+                    assert mm is None, f"strand level data differs {mm}"
+            hair.IncreaseSimulationFrame()
+
+
+@pytest.mark.parametrize("V,F,wind", [(16, 0, False), (32, 4, True), (64, 0, True), (8, 2, True)])
+def test_substeps_vs_oracles(gpu, V, F, wind):
+    """Simulate() (reference dispatch structure) stepwise vs FP32 (bit-exact)
+    and FP64 (<= 1e-5 relative) over several frames."""
+    print(f"[substeps] V={V} F={F} wind={wind}", flush=True)
+    wl = small_workload(128, V, F, wind=wind)
+    hair = make_hair(wl)
+    sim = tfx.TressFXSimulation()
+    o32, o64 = oracle_pair(wl.asset)
+    worst = 0.0
+    for frame in range(8):
+        for sub in range(wl.substeps):
+            set_substep_inputs(hair, wl, frame, sub)
+            params, bones = hair.simulation_params()
+            st0 = hair.download_state()
+            o32.set_state(st0)
+            o64.set_state(st0)
+            sim.Simulate([hair])
+            o32.simulate(params, bones)
+            o64.simulate(params, bones)
+            st = hair.download_state()
+            compare_state_fp32(st, o32, f"frame {frame} sub {sub}")
+            for k in ("pos", "prev"):
+                e = rel_err(st[k], getattr(o64, k))
+                worst = max(worst, float(e.max()))
+                assert e.max() <= REL_TOL, f"frame {frame} sub {sub}: {k} rel err {e.max():.3e} at {int(e.argmax())}"
+    print(f"worst relative error vs FP64: {worst:.3e}")
+
+
+def test_reset_and_frame_counter(gpu):
+    wl = small_workload(64, 16, 0)
+    hair = make_hair(wl)
+    sim = tfx.TressFXSimulation()
+    for frame in range(3):
+        set_substep_inputs(hair, wl, frame, 0)
+        p, _ = hair.simulation_params()
+        assert p.reset_positions == (1.0 if hair.info().simulation_frame < 2 else 0.0)
+        sim.Simulate([hair])
+    assert hair.info().simulation_frame == 3
+    hair.ResetPositions()
+    p, _ = hair.simulation_params()
+    assert p.reset_positions == 1.0
+
+
+def _mesh_setup(torch, G=256, V=32, F=2, grid_n=64, cell=0.005, subdiv=3, seed=11):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    roots, nrm = synth.hemisphere_roots(G, 0.1, rng)
+    strands = synth.grow_strands(roots, nrm, V, 0.005, 0.15, rng)
+    asset = synth.build_asset(strands, F, rng, follow_sigma=0.002)
+    mesh = synth.NoisySphereMesh(rng, radius=0.1, subdivisions=subdiv, amp=0.01)
+    wl = synth.Workload(asset, synth.settings_config2(), 2, synth.head_yaw, mesh=mesh,
+                        sdf_grid=((-0.5 * grid_n * cell,) * 3, cell, (grid_n,) * 3), wind_fn=synth.wind_gust,
+                        collision_margin=3.0)   # 15 mm: vertices near the roots lie inside it
+    dm = DeviceMesh(torch, mesh, synth.head_yaw)
+    return wl, dm
+
+
+def test_sdf_build_vs_oracle(gpu):
+    torch = gpu
+    wl, dm = _mesh_setup(torch)
+    desc = dm.update(0.25)
+    origin, cell, dims = wl.sdf_grid
+    sdf = tfx.TressFXSDFCollision(desc, collision_margin=0.5, explicit=(origin, cell, dims))
+    system = tfx.TressFXSDFCollisionSystem()
+    sdf.Update(system)
+    g = sdf.download()
+    ref32 = oracle.sdf_build(dm.verts_h, dm.mesh.indices, origin, np.float32(cell), dims, np.float32)
+    mm = first_mismatch(g, ref32)
+    assert mm is None, f"SDF differs from the FP32 oracle at cell {mm[0]}: gpu={mm[1]} oracle={mm[2]} ({mm[3]} cells)"
+    ref64 = oracle.sdf_build(dm.verts_h.astype(np.float64), dm.mesh.indices, np.asarray(origin, np.float64),
+                             float(np.float32(cell)), dims, np.float64)
+    band = (g != np.float32(1e10)) & (ref64 != float(np.float32(1e10)))
+    assert np.array_equal(g == np.float32(1e10), ref64 == float(np.float32(1e10)))
+    assert np.abs(g[band] - ref64[band]).max() <= 1e-6
+    assert band.sum() > 1000
+
+
+def test_sdf_reference_grid_sizing(gpu):
+    """The numCellsInX constructor (TressFXSDFCollision.cpp:38-86) vs its transcription."""
+    torch = gpu
+    wl, dm = _mesh_setup(torch)
+    desc = dm.update(0.0)
+    sdf = tfx.TressFXSDFCollision(desc, num_cells_in_x=40, collision_margin=0.5)
+    gi = sdf.grid()
+    lib = oracle.load()
+    import ctypes
+    o = (ctypes.c_float * 3)()
+    c = ctypes.c_float()
+    d = (ctypes.c_int * 3)()
+    tot = ctypes.c_int()
+    mn = (ctypes.c_float * 3)(*desc.initial_bbox_min)
+    mx = (ctypes.c_float * 3)(*desc.initial_bbox_max)
+    cur = (ctypes.c_float * 3)(*desc.bbox_min)
+    lib.orc_sdf_grid_from_bbox(ctypes.addressof(mn), ctypes.addressof(mx), ctypes.addressof(cur), 40,
+                               ctypes.addressof(o), ctypes.addressof(c), ctypes.addressof(d), ctypes.addressof(tot))
+    assert (gi.num_cells_x, gi.num_cells_y, gi.num_cells_z) == tuple(d)
+    assert gi.num_total_cells == tot.value
+    assert gi.cell_size == c.value
+    assert tuple(gi.origin) == tuple(o)
+    system = tfx.TressFXSDFCollisionSystem()
+    desc2 = dm.update(0.5)
+    sdf.SetMesh(desc2)
+    sdf.Update(system)
+    g = sdf.download()
+    gi = sdf.grid()
+    ref = oracle.sdf_build(dm.verts_h, dm.mesh.indices, np.array(gi.origin, np.float32), np.float32(gi.cell_size),
+                           (gi.num_cells_x, gi.num_cells_y, gi.num_cells_z), np.float32)
+    assert bits_equal(g, ref)
+
+
+def test_collide_and_fused_vs_oracle(gpu):
+    """Plain Simulate + CollideWithHair and the fused substep, stepwise against
+    the FP32 oracle (bit-exact) and the FP64 oracle (1e-5), with SDF, wind and
+    follow strands."""
+    torch = gpu
+    wl, dm = _mesh_setup(torch)
+    origin, cell, dims = wl.sdf_grid
+    system = tfx.TressFXSDFCollisionSystem()
+    sim = tfx.TressFXSimulation()
+    hair_a = tfx.TressFXHairObject(wl.asset)   # plain dispatch
+    hair_b = tfx.TressFXHairObject(wl.asset)   # fused
+    desc = dm.update(0.0)
+    sdf = tfx.TressFXSDFCollision(desc, collision_margin=wl.collision_margin, explicit=(origin, cell, dims))
+    o32, o64 = oracle_pair(wl.asset)
+    G, F, V = wl.asset.num_guide_strands, wl.asset.num_follow_strands_per_guide, wl.asset.num_vertices_per_strand
+    guide_v01 = ((np.arange(G) * (F + 1))[:, None] * V + np.arange(2)[None, :]).reshape(-1)
+    hits = 0
+    for frame in range(6):
+        desc = dm.update(frame / synth.FRAME_RATE)
+        sdf.SetMesh(desc)
+        sdf.Update(system)
+        grid32 = sdf.download()
+        for sub in range(wl.substeps):
+            for h in (hair_a, hair_b):
+                set_substep_inputs(h, wl, frame, sub)
+            params, bones = hair_a.simulation_params()
+            st0 = hair_a.download_state()
+            hair_b.upload_state(st0)
+            o32.set_state(st0)
+            o64.set_state(st0)
+            sim.Simulate([hair_a])
+            sdf.CollideWithHair(system, hair_a)
+            sim.SimulateFused([hair_b], [sdf])
+            o32.simulate(params, bones)
+            o64.simulate(params, bones)
+            cp = sdf.collision_params(hair_a)
+            before = o32.pos.copy()
+            oracle.sdf_collide(o32.pos, o32.prev, V, grid32, np.array(cp.origin[:3], np.float32), np.float32(cp.cell_size),
+                               dims, np.float32(cp.collision_margin))
+            hits += int((before != o32.pos).any(axis=1).sum())
+            oracle.sdf_collide(o64.pos, o64.prev, V, grid32.astype(np.float64),
+                               np.array(cp.origin[:3], np.float64), float(cp.cell_size), dims,
+                               float(cp.collision_margin))
+            sa = hair_a.download_state()
+            sb = hair_b.download_state()
+            compare_state_fp32(sa, o32, f"plain frame {frame} sub {sub}")
+            compare_state_fp32(sb, o32, f"fused frame {frame} sub {sub}", keys=("pos", "prev", "tangent"))
+            assert bits_equal(sb["prevprev"][guide_v01], o32.prevprev[guide_v01])
+            for k in ("pos", "prev"):
+                e = rel_err(sa[k], getattr(o64, k))
+                assert e.max() <= REL_TOL, f"frame {frame} sub {sub}: {k} rel err {e.max():.3e}"
+    assert hits > 0, "the SDF never moved a vertex: the test does not exercise the collision"
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_fused_multi_object_mixed_lengths(gpu, variant):
+    """Several objects of different strand lengths in one fused call (one
+    persistent launch per length class) equal their plain results."""
+    sim_p = tfx.TressFXSimulation()
+    sim_f = tfx.TressFXSimulation()
+    objs = []
+    for i, (G, V, F) in enumerate([(50, 16, 0), (300, 32, 1), (70, 64, 0), (33, 8, 2), (129, 4, 0)]):
+        wl = small_workload(G, V, F, seed=100 + i)
+        objs.append((wl, make_hair(wl), make_hair(wl)))
+    for V in (4, 8, 16, 32, 64):
+        sim_f.SetLaunchShape(V, variant, 2 if variant else 0)
+    for frame in range(3):
+        for sub in range(2):
+            for wl, a, b in objs:
+                set_substep_inputs(a, wl, frame, sub)
+                set_substep_inputs(b, wl, frame, sub)
+            sim_p.Simulate([a for _, a, _ in objs])
+            sim_f.SimulateFused([b for _, _, b in objs])
+            for wl, a, b in objs:
+                sa, sb = a.download_state(), b.download_state()
+                for k in ("pos", "prev", "tangent"):
+                    mm = first_mismatch(sb[k], sa[k])
+                    assert mm is None, f"V={wl.asset.num_vertices_per_strand} {k}: {mm}"

@@ -1,0 +1,507 @@
+"""MI355X-native TressFX 4.1 strand simulation and SDF collision.
+
+Python mirror of the reference's engine interface over the C-ABI of
+``include/tressfx_amd.h`` (the HIP library ``libtressfx_amd.so``):
+
+    TressFXSimulation            src/TressFX/TressFXSimulation.h:34-48
+    TressFXHairObject            src/TressFX/TressFXHairObject.h:85-191
+    TressFXSDFCollisionSystem    src/TressFX/TressFXSDFCollision.h:40-56
+    TressFXSDFCollision          src/TressFX/TressFXSDFCollision.h:58-124
+    TressFXSimulationSettings    src/TressFX/TressFXSettings.h:35-87
+
+Every compute call goes to the HIP library; if it is missing the import of the
+library raises (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libtressfx_amd.so")
+
+TFX_OK = 0
+TFX_MAX_NUM_BONES = 512
+TFX_MAX_NUM_COLLISION_CAPSULES = 8
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class TFXError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tressfx_amd error {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------
+# ABI records (layouts of include/tressfx_amd.h)
+# ---------------------------------------------------------------------------
+class SimulationSettings(ctypes.Structure):
+    _fields_ = [
+        ("vsp_coeff", ctypes.c_float),
+        ("vsp_accel_threshold", ctypes.c_float),
+        ("local_constraint_stiffness", ctypes.c_float),
+        ("local_constraints_iterations", ctypes.c_int),
+        ("global_constraint_stiffness", ctypes.c_float),
+        ("global_constraints_range", ctypes.c_float),
+        ("length_constraints_iterations", ctypes.c_int),
+        ("damping", ctypes.c_float),
+        ("gravity_magnitude", ctypes.c_float),
+        ("tip_separation", ctypes.c_float),
+        ("wind_magnitude", ctypes.c_float),
+        ("wind_direction", ctypes.c_float * 3),
+        ("wind_angle_radians", ctypes.c_float),
+        ("clamp_position_delta", ctypes.c_float),
+    ]
+
+    @classmethod
+    def defaults(cls):
+        """TressFXSimulationSettings() defaults, TressFXSettings.h:38-56."""
+        s = cls()
+        s.vsp_coeff = 0.8
+        s.vsp_accel_threshold = 1.4
+        s.local_constraint_stiffness = 0.9
+        s.local_constraints_iterations = 2
+        s.global_constraint_stiffness = 0.0
+        s.global_constraints_range = 0.0
+        s.length_constraints_iterations = 2
+        s.damping = 0.08
+        s.gravity_magnitude = 0.0
+        s.tip_separation = 0.0
+        s.wind_magnitude = 0.0
+        s.wind_direction[0], s.wind_direction[1], s.wind_direction[2] = 1.0, 0.0, 0.0
+        s.wind_angle_radians = np.float32(3.1415926 / 180.0 * 40.0)
+        s.clamp_position_delta = 20.0
+        return s
+
+
+TressFXSimulationSettings = SimulationSettings
+
+
+class SimulationParams(ctypes.Structure):
+    _fields_ = [
+        ("wind", ctypes.c_float * 4),
+        ("wind1", ctypes.c_float * 4),
+        ("wind2", ctypes.c_float * 4),
+        ("wind3", ctypes.c_float * 4),
+        ("shape", ctypes.c_float * 4),
+        ("grav_time_tip", ctypes.c_float * 4),
+        ("sim_ints", ctypes.c_int * 4),
+        ("counts", ctypes.c_int * 4),
+        ("vsp", ctypes.c_float * 4),
+        ("reset_positions", ctypes.c_float),
+        ("clamp_position_delta", ctypes.c_float),
+        ("pad1", ctypes.c_float),
+        ("pad2", ctypes.c_float),
+        ("center_and_radius0", (ctypes.c_float * 4) * TFX_MAX_NUM_COLLISION_CAPSULES),
+        ("center_and_radius1", (ctypes.c_float * 4) * TFX_MAX_NUM_COLLISION_CAPSULES),
+        ("num_collision_capsules", ctypes.c_int * 4),
+        ("cpu_local_shape_iterations", ctypes.c_int),
+        ("num_bones", ctypes.c_int),
+        ("simulation_frame", ctypes.c_int),
+        ("pad3", ctypes.c_int),
+    ]
+
+
+class AssetDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_guide_strands", ctypes.c_int),
+        ("num_follow_strands_per_guide", ctypes.c_int),
+        ("num_vertices_per_strand", ctypes.c_int),
+        ("pad", ctypes.c_int),
+        ("positions", c_float_p),
+        ("tangents", c_float_p),
+        ("rest_lengths", c_float_p),
+        ("follow_root_offsets", c_float_p),
+        ("bone_skinning_data", ctypes.c_void_p),
+    ]
+
+
+class HairInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "num_guide_strands", "num_follow_strands_per_guide", "num_vertices_per_strand", "num_total_strands",
+        "num_total_vertices", "simulation_frame", "cpu_local_shape_iterations", "pad")]
+
+
+class DynamicState(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_void_p),
+        ("positions_prev", ctypes.c_void_p),
+        ("positions_prev_prev", ctypes.c_void_p),
+        ("tangents", ctypes.c_void_p),
+        ("strand_level_data", ctypes.c_void_p),
+    ]
+
+
+class SdfMeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("vertices", ctypes.c_void_p),
+        ("indices", ctypes.c_void_p),
+        ("num_vertices", ctypes.c_int),
+        ("num_triangles", ctypes.c_int),
+        ("bbox_min", ctypes.c_float * 3),
+        ("bbox_max", ctypes.c_float * 3),
+        ("initial_bbox_min", ctypes.c_float * 3),
+        ("initial_bbox_max", ctypes.c_float * 3),
+    ]
+
+
+class SdfGridInfo(ctypes.Structure):
+    _fields_ = [
+        ("origin", ctypes.c_float * 3),
+        ("cell_size", ctypes.c_float),
+        ("num_cells_x", ctypes.c_int),
+        ("num_cells_y", ctypes.c_int),
+        ("num_cells_z", ctypes.c_int),
+        ("num_total_cells", ctypes.c_int),
+        ("collision_margin", ctypes.c_float),
+        ("padding_boundary", ctypes.c_float),
+        ("explicit_grid", ctypes.c_int),
+        ("pad", ctypes.c_int),
+    ]
+
+
+class SdfCollisionParams(ctypes.Structure):
+    _fields_ = [
+        ("origin", ctypes.c_float * 4),
+        ("cell_size", ctypes.c_float),
+        ("num_cells_x", ctypes.c_int),
+        ("num_cells_y", ctypes.c_int),
+        ("num_cells_z", ctypes.c_int),
+        ("max_marching_cubes_vertices", ctypes.c_int),
+        ("marching_cubes_isolevel", ctypes.c_float),
+        ("collision_margin", ctypes.c_float),
+        ("num_hair_vertices_per_strand", ctypes.c_int),
+        ("num_total_hair_vertices", ctypes.c_int),
+        ("pad1", ctypes.c_float),
+        ("pad2", ctypes.c_float),
+        ("pad3", ctypes.c_float),
+    ]
+
+
+# Symbols of include/tressfx_amd.h: name -> (restype, argtypes)
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+EXPORTS = {
+    "tfx_last_error": (ctypes.c_char_p, []),
+    "tfx_get_version": (_I, [ctypes.POINTER(_I)] * 3),
+    "tfx_init": (_I, [_I]),
+    "tfx_simulation_settings_default": (None, [ctypes.POINTER(SimulationSettings)]),
+    "tfx_hair_create": (_I, [ctypes.POINTER(AssetDesc), ctypes.POINTER(_VP)]),
+    "tfx_hair_destroy": (_I, [_VP]),
+    "tfx_hair_update_bone_matrices": (_I, [_VP, c_float_p, _I]),
+    "tfx_hair_update_simulation_parameters": (_I, [_VP, ctypes.POINTER(SimulationSettings), _F]),
+    "tfx_hair_set_collision_capsules": (_I, [_VP, c_float_p, c_float_p, _I]),
+    "tfx_hair_reset_positions": (_I, [_VP]),
+    "tfx_hair_increase_simulation_frame": (_I, [_VP]),
+    "tfx_hair_get_info": (_I, [_VP, ctypes.POINTER(HairInfo)]),
+    "tfx_hair_get_simulation_params": (_I, [_VP, ctypes.POINTER(SimulationParams), c_float_p]),
+    "tfx_hair_get_dynamic_state": (_I, [_VP, ctypes.POINTER(DynamicState)]),
+    "tfx_hair_download_state": (_I, [_VP, _VP, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p]),
+    "tfx_hair_upload_state": (_I, [_VP, _VP, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p]),
+    "tfx_simulation_create": (_I, [ctypes.POINTER(_VP)]),
+    "tfx_simulation_destroy": (_I, [_VP]),
+    "tfx_simulation_simulate": (_I, [_VP, _VP, ctypes.POINTER(_VP), _I]),
+    "tfx_simulation_run_stage": (_I, [_VP, _VP, _VP, _I]),
+    "tfx_simulation_simulate_fused": (_I, [_VP, _VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I]),
+    "tfx_simulation_set_launch_shape": (_I, [_VP, _I, _I, _I]),
+    "tfx_sdf_collision_system_create": (_I, [ctypes.POINTER(_VP)]),
+    "tfx_sdf_collision_system_destroy": (_I, [_VP]),
+    "tfx_sdf_create": (_I, [ctypes.POINTER(SdfMeshDesc), _I, _F, ctypes.POINTER(_VP)]),
+    "tfx_sdf_create_explicit": (_I, [ctypes.POINTER(SdfMeshDesc), c_float_p, _F, _I, _I, _I, _F, ctypes.POINTER(_VP)]),
+    "tfx_sdf_destroy": (_I, [_VP]),
+    "tfx_sdf_set_mesh": (_I, [_VP, ctypes.POINTER(SdfMeshDesc)]),
+    "tfx_sdf_update": (_I, [_VP, _VP, _VP]),
+    "tfx_sdf_collide_with_hair": (_I, [_VP, _VP, _VP, _VP]),
+    "tfx_sdf_get_grid": (_I, [_VP, ctypes.POINTER(SdfGridInfo)]),
+    "tfx_sdf_get_data_buffer": (_I, [_VP, ctypes.POINTER(_VP)]),
+    "tfx_sdf_download": (_I, [_VP, _VP, c_float_p]),
+    "tfx_sdf_get_collision_params": (_I, [_VP, _VP, ctypes.POINTER(SdfCollisionParams)]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libtressfx_amd.so and binds every exported symbol. Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python -m tressfx_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != TFX_OK:
+        msg = _lib.tfx_last_error()
+        raise TFXError(rc, msg.decode() if msg else "")
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(c_float_p)
+
+
+def init(device=0):
+    lib = load_library()
+    _check(lib.tfx_init(device))
+    return lib
+
+
+# ---------------------------------------------------------------------------
+# Host asset arrays (TressFXAsset, src/TressFX/TressFXAsset.h:50-100)
+# ---------------------------------------------------------------------------
+class HairAsset:
+    """Processed asset arrays: positions/tangents float32[S*V,4], rest_lengths
+    float32[S*V], follow_root_offsets float32[S,4], bone_skinning_data
+    float32[S,8] (4 indices as float, 4 weights)."""
+
+    def __init__(self, num_guide_strands, num_follow_strands_per_guide, num_vertices_per_strand, positions,
+                 tangents, rest_lengths, follow_root_offsets, bone_skinning_data):
+        self.num_guide_strands = int(num_guide_strands)
+        self.num_follow_strands_per_guide = int(num_follow_strands_per_guide)
+        self.num_vertices_per_strand = int(num_vertices_per_strand)
+        self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 4)
+        self.tangents = np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 4)
+        self.rest_lengths = np.ascontiguousarray(rest_lengths, dtype=np.float32).reshape(-1)
+        self.follow_root_offsets = np.ascontiguousarray(follow_root_offsets, dtype=np.float32).reshape(-1, 4)
+        self.bone_skinning_data = np.ascontiguousarray(bone_skinning_data, dtype=np.float32).reshape(-1, 8)
+
+    @property
+    def num_total_strands(self):
+        return self.num_guide_strands * (self.num_follow_strands_per_guide + 1)
+
+    @property
+    def num_total_vertices(self):
+        return self.num_total_strands * self.num_vertices_per_strand
+
+    def desc(self):
+        d = AssetDesc()
+        d.num_guide_strands = self.num_guide_strands
+        d.num_follow_strands_per_guide = self.num_follow_strands_per_guide
+        d.num_vertices_per_strand = self.num_vertices_per_strand
+        d.positions = _fptr(self.positions)
+        d.tangents = _fptr(self.tangents)
+        d.rest_lengths = _fptr(self.rest_lengths)
+        d.follow_root_offsets = _fptr(self.follow_root_offsets)
+        d.bone_skinning_data = self.bone_skinning_data.ctypes.data
+        return d
+
+
+# ---------------------------------------------------------------------------
+# Engine-interface mirror
+# ---------------------------------------------------------------------------
+class TressFXHairObject:
+    """TressFXHairObject(TressFXAsset*, ...), TressFXHairObject.cpp:44-164."""
+
+    def __init__(self, asset: HairAsset):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        desc = asset.desc()
+        _check(self._lib.tfx_hair_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+        self.num_guide_strands = asset.num_guide_strands
+        self.num_follow = asset.num_follow_strands_per_guide
+        self.V = asset.num_vertices_per_strand
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_hair_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def UpdateBoneMatrices(self, matrices):
+        m = np.ascontiguousarray(matrices, dtype=np.float32).reshape(-1, 16)
+        _check(self._lib.tfx_hair_update_bone_matrices(self.handle, _fptr(m), m.shape[0]))
+
+    def UpdateSimulationParameters(self, settings: SimulationSettings, time_step):
+        _check(self._lib.tfx_hair_update_simulation_parameters(self.handle, ctypes.byref(settings),
+                                                               ctypes.c_float(time_step)))
+
+    def SetCollisionCapsules(self, center_and_radius0, center_and_radius1):
+        c0 = np.ascontiguousarray(center_and_radius0, dtype=np.float32).reshape(-1, 4)
+        c1 = np.ascontiguousarray(center_and_radius1, dtype=np.float32).reshape(-1, 4)
+        assert c0.shape == c1.shape
+        _check(self._lib.tfx_hair_set_collision_capsules(self.handle, _fptr(c0), _fptr(c1), c0.shape[0]))
+
+    def ResetPositions(self):
+        _check(self._lib.tfx_hair_reset_positions(self.handle))
+
+    def IncreaseSimulationFrame(self):
+        _check(self._lib.tfx_hair_increase_simulation_frame(self.handle))
+
+    def info(self):
+        o = HairInfo()
+        _check(self._lib.tfx_hair_get_info(self.handle, ctypes.byref(o)))
+        return o
+
+    def GetNumTotalHairVertices(self):
+        return self.info().num_total_vertices
+
+    def GetNumTotalHairStrands(self):
+        return self.info().num_total_strands
+
+    def GetNumVerticesPerStrand(self):
+        return self.V
+
+    def GetNumFollowHairsPerGuideHair(self):
+        return self.num_follow
+
+    def GetCPULocalShapeIterations(self):
+        return self.info().cpu_local_shape_iterations
+
+    def simulation_params(self):
+        """The packed parameter block and the 512-entry bone table."""
+        p = SimulationParams()
+        bones = np.zeros((TFX_MAX_NUM_BONES, 16), dtype=np.float32)
+        _check(self._lib.tfx_hair_get_simulation_params(self.handle, ctypes.byref(p), _fptr(bones)))
+        return p, bones
+
+    def GetDynamicState(self):
+        o = DynamicState()
+        _check(self._lib.tfx_hair_get_dynamic_state(self.handle, ctypes.byref(o)))
+        return o
+
+    def download_state(self, stream=None):
+        n = self.GetNumTotalHairVertices()
+        s = self.GetNumTotalHairStrands()
+        st = {k: np.zeros((n, 4), np.float32) for k in ("pos", "prev", "prevprev", "tangent")}
+        st["strand_level"] = np.zeros((s, 12), np.float32)
+        _check(self._lib.tfx_hair_download_state(self.handle, stream, _fptr(st["pos"]), _fptr(st["prev"]),
+                                                 _fptr(st["prevprev"]), _fptr(st["tangent"]),
+                                                 _fptr(st["strand_level"])))
+        return st
+
+    def upload_state(self, state, stream=None):
+        keep = {k: np.ascontiguousarray(state[k], np.float32) for k in state if state[k] is not None}
+        p = lambda k: _fptr(keep[k]) if k in keep else None
+        _check(self._lib.tfx_hair_upload_state(self.handle, stream, p("pos"), p("prev"), p("prevprev"),
+                                               p("tangent"), p("strand_level")))
+
+
+class TressFXSimulation:
+    """TressFXSimulation, TressFXSimulation.h:34-48."""
+
+    def __init__(self):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.tfx_simulation_create(ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_simulation_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Simulate(self, hair_objects, stream=None):
+        arr = (ctypes.c_void_p * len(hair_objects))(*[h.handle for h in hair_objects])
+        _check(self._lib.tfx_simulation_simulate(self.handle, stream, arr, len(hair_objects)))
+
+    def RunStage(self, hair, stage, stream=None):
+        _check(self._lib.tfx_simulation_run_stage(self.handle, stream, hair.handle, stage))
+
+    def SetLaunchShape(self, vertices_per_strand, variant, blocks_per_cu=0):
+        _check(self._lib.tfx_simulation_set_launch_shape(self.handle, vertices_per_strand, variant, blocks_per_cu))
+
+    def SimulateFused(self, hair_objects, sdfs=None, stream=None):
+        n = len(hair_objects)
+        arr = (ctypes.c_void_p * n)(*[h.handle for h in hair_objects])
+        sarr = None
+        if sdfs is not None:
+            sarr = (ctypes.c_void_p * n)(*[(s.handle if s is not None else None) for s in sdfs])
+        _check(self._lib.tfx_simulation_simulate_fused(self.handle, stream, arr, sarr, n))
+
+
+class TressFXSDFCollisionSystem:
+    def __init__(self):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.tfx_sdf_collision_system_create(ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_sdf_collision_system_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TressFXSDFCollision:
+    """TressFXSDFCollision, TressFXSDFCollision.h:58-124.  `mesh` is an
+    SdfMeshDesc whose vertex/index pointers are device buffers."""
+
+    def __init__(self, mesh: SdfMeshDesc, num_cells_in_x=None, collision_margin=0.0, explicit=None):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        if explicit is not None:
+            origin, cell, dims = explicit
+            o = np.asarray(origin, np.float32)
+            _check(self._lib.tfx_sdf_create_explicit(ctypes.byref(mesh), _fptr(o), ctypes.c_float(cell),
+                                                     int(dims[0]), int(dims[1]), int(dims[2]),
+                                                     ctypes.c_float(collision_margin), ctypes.byref(h)))
+        else:
+            _check(self._lib.tfx_sdf_create(ctypes.byref(mesh), int(num_cells_in_x), ctypes.c_float(collision_margin),
+                                            ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_sdf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def SetMesh(self, mesh: SdfMeshDesc):
+        _check(self._lib.tfx_sdf_set_mesh(self.handle, ctypes.byref(mesh)))
+
+    def Update(self, system, stream=None):
+        _check(self._lib.tfx_sdf_update(self.handle, system.handle, stream))
+
+    def CollideWithHair(self, system, hair, stream=None):
+        _check(self._lib.tfx_sdf_collide_with_hair(self.handle, system.handle, stream, hair.handle))
+
+    def grid(self):
+        g = SdfGridInfo()
+        _check(self._lib.tfx_sdf_get_grid(self.handle, ctypes.byref(g)))
+        return g
+
+    def collision_params(self, hair):
+        p = SdfCollisionParams()
+        _check(self._lib.tfx_sdf_get_collision_params(self.handle, hair.handle, ctypes.byref(p)))
+        return p
+
+    def download(self, stream=None):
+        g = self.grid()
+        out = np.zeros(g.num_cells_x * g.num_cells_y * g.num_cells_z, np.float32)
+        _check(self._lib.tfx_sdf_download(self.handle, stream, _fptr(out)))
+        return out

@@ -1,0 +1,862 @@
+// tfx_capi.hip — the C-ABI of include/tressfx_amd.h: object lifetimes, the
+// host-side parameter packing of TressFXHairObject, and the dispatch sequences
+// of TressFXSimulation and TressFXSDFCollision over HIP streams.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tfx_device.h"
+
+using tfx::DevHair;
+using tfx::DevParams;
+using tfx::SdfGrid;
+
+extern "C" hipError_t tfx_launch_stage(int stage, const tfx::DevHair* h, hipStream_t stream);
+extern "C" hipError_t tfx_launch_sdf_build(uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
+                                           const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream);
+extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, int V, const float* sdf,
+                                             tfx::SdfGrid g, float margin, hipStream_t stream);
+extern "C" hipError_t tfx_launch_fused(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
+                                       int nitems, int grid, hipStream_t stream);
+extern "C" int tfx_fused_block_vertices(int V, int variant);
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+static thread_local int g_device = 0;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(TFX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_ARG(cond, msg)                                            \
+    do {                                                                \
+        if (!(cond)) return fail(TFX_ERR_INVALID_ARGUMENT, "%s", msg);  \
+    } while (0)
+
+template <class T> static hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+// ---------------------------------------------------------------------------
+// Pinned staging ring: host writes slot k while slot k-1's copy may still be
+// in flight (the role of the reference's double-buffered m_SimCB[2]).
+// ---------------------------------------------------------------------------
+struct Staging {
+    void* host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    int next = 0;
+
+    int ensure(size_t n) {
+        if (n <= bytes) return TFX_OK;
+        release();
+        size_t cap = std::max<size_t>(n, 4096);
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(hipHostMalloc(&host[k], cap, hipHostMallocDefault));
+            HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        }
+        bytes = cap;
+        return TFX_OK;
+    }
+    // Returns a host slot that is free to overwrite.
+    int acquire(void** out, int* slot) {
+        int k = next;
+        next ^= 1;
+        HIP_TRY(hipEventSynchronize(ev[k]));
+        *out = host[k];
+        *slot = k;
+        return TFX_OK;
+    }
+    int release_slot(int k, hipStream_t s) {
+        HIP_TRY(hipEventRecord(ev[k], s));
+        return TFX_OK;
+    }
+    void release() {
+        for (int k = 0; k < 2; ++k) {
+            if (ev[k]) { hipEventSynchronize(ev[k]); hipEventDestroy(ev[k]); ev[k] = nullptr; }
+            if (host[k]) { hipHostFree(host[k]); host[k] = nullptr; }
+        }
+        bytes = 0;
+    }
+    ~Staging() { release(); }
+};
+
+// ---------------------------------------------------------------------------
+// Objects
+// ---------------------------------------------------------------------------
+struct tfx_hair {
+    int G = 0, F = 0, V = 0, S = 0, N = 0;
+    int device = 0;
+    float4* pos = nullptr;
+    float4* prev = nullptr;
+    float4* prevprev = nullptr;
+    float4* tangent = nullptr;
+    tfx_strand_level_data* sld = nullptr;
+    float4* rest = nullptr;
+    float* restlen = nullptr;
+    float4* follow_off = nullptr;
+    tfx_bone_skinning_data* skin = nullptr;
+    float* bones_d = nullptr;
+    DevParams* params_d = nullptr;
+
+    tfx_simulation_params params{};       // m_SimCB (single block: kernel arguments are copied at launch)
+    std::vector<float> bones_h;           // 512 x 16
+    int bones_dirty = 0;                  // leading matrices not yet on the device
+    int frame = 0;                        // m_SimulationFrame
+    int tangent_w_uniform = 1;
+    float tangent_w = 0.0f;
+    Staging staging;
+};
+
+struct tfx_simulation {
+    int device = 0;
+    Staging staging;
+    void* table_d = nullptr;   // DevHair[] followed by WorkItem[]
+    size_t table_bytes = 0;
+    int variant[65];           // launch-shape variant per V (autotuned)
+    int grid_per_cu = 4;
+};
+
+struct tfx_sdf_collision_system {
+    int device = 0;
+};
+
+struct tfx_sdf {
+    tfx_sdf_mesh_desc mesh{};
+    float origin[3] = {0, 0, 0};
+    float cell = 0;
+    int nx = 0, ny = 0, nz = 0;
+    int total_cells = 0;     // allocation
+    float margin = 0;        // in cell units
+    float pad_boundary = 0;
+    int explicit_grid = 0;
+    uint32_t* cells = nullptr;
+    int device = 0;
+};
+
+static void set_identity_bones(std::vector<float>& b) {
+    b.assign(16 * TFX_MAX_NUM_BONES, 0.0f);
+    for (int i = 0; i < TFX_MAX_NUM_BONES; ++i)
+        for (int k = 0; k < 4; ++k) b[16 * i + 5 * k] = 1.0f;
+}
+
+static void hair_free(tfx_hair* h) {
+    hipFree(h->pos); hipFree(h->prev); hipFree(h->prevprev); hipFree(h->tangent);
+    hipFree(h->sld); hipFree(h->rest); hipFree(h->restlen); hipFree(h->follow_off);
+    hipFree(h->skin); hipFree(h->bones_d); hipFree(h->params_d);
+}
+
+static void compute_tangent_w(tfx_hair* h, const float* tangents) {
+    h->tangent_w_uniform = 1;
+    h->tangent_w = 0.0f;
+    if (!tangents) return;
+    bool first = true;
+    uint32_t w0 = 0;
+    for (int g = 0; g < h->G; ++g) {
+        const size_t base = (size_t)g * (h->F + 1) * h->V;
+        for (int v = 0; v < h->V; ++v) {
+            uint32_t w;
+            std::memcpy(&w, &tangents[4 * (base + v) + 3], 4);
+            if (first) { w0 = w; first = false; }
+            else if (w != w0) { h->tangent_w_uniform = 0; return; }
+        }
+    }
+    std::memcpy(&h->tangent_w, &w0, 4);
+}
+
+// DevParams from the packed reference parameter block.
+static void pack_dev_params(const tfx_hair* h, const tfx_sdf* sdf, DevParams& d) {
+    const tfx_simulation_params& p = h->params;
+    std::memset(&d, 0, sizeof(d));
+    d.wind0 = make_float4(p.wind[0], p.wind[1], p.wind[2], p.wind[3]);
+    d.wind1 = make_float4(p.wind1[0], p.wind1[1], p.wind1[2], p.wind1[3]);
+    d.wind2 = make_float4(p.wind2[0], p.wind2[1], p.wind2[2], p.wind2[3]);
+    d.wind3 = make_float4(p.wind3[0], p.wind3[1], p.wind3[2], p.wind3[3]);
+    d.damping = p.shape[0];
+    d.local_stiffness = p.shape[1];
+    d.global_stiffness = p.shape[2];
+    d.global_range = p.shape[3];
+    d.gravity = p.grav_time_tip[0];
+    d.dt = p.grav_time_tip[1];
+    d.tip_sep = p.grav_time_tip[2];
+    // Integrate's decay = exp(-dampingCoeff * g_TimeStep * 60) (TressFXSimulation.hlsl:443), once per substep.
+    d.decay = std::exp((-p.shape[0] * p.grav_time_tip[1]) * 60.0f);
+    d.length_iters = p.sim_ints[0];
+    d.lsc_passes = p.cpu_local_shape_iterations;
+    d.wind_on = (p.wind[0] != 0.0f || p.wind[1] != 0.0f || p.wind[2] != 0.0f) ? 1 : 0;
+    d.num_capsules = std::min(std::max(p.num_collision_capsules[0], 0), TFX_MAX_NUM_COLLISION_CAPSULES);
+    d.vsp_coeff = p.vsp[0];
+    d.vsp_accel_threshold = p.vsp[1];
+    d.reset = p.reset_positions;
+    d.clamp = p.clamp_position_delta;
+    for (int c = 0; c < TFX_MAX_NUM_COLLISION_CAPSULES; ++c) {
+        d.cap0[c] = make_float4(p.center_and_radius0[c][0], p.center_and_radius0[c][1], p.center_and_radius0[c][2],
+                                p.center_and_radius0[c][3]);
+        d.cap1[c] = make_float4(p.center_and_radius1[c][0], p.center_and_radius1[c][1], p.center_and_radius1[c][2],
+                                p.center_and_radius1[c][3]);
+    }
+    if (sdf) {
+        d.sdf_ox = sdf->origin[0]; d.sdf_oy = sdf->origin[1]; d.sdf_oz = sdf->origin[2];
+        d.sdf_cell = sdf->cell;
+        d.sdf_nx = sdf->nx; d.sdf_ny = sdf->ny; d.sdf_nz = sdf->nz;
+        d.has_sdf = 1;
+        // m_CollisionMargin * m_CellSize (TressFXSDFCollision.cpp:194)
+        d.sdf_margin = sdf->margin * sdf->cell;
+    }
+    d.tangent_w_uniform = h->tangent_w_uniform;
+    d.tangent_w = h->tangent_w;
+}
+
+// TressFXHairObject::UpdateConstantBuffer: stream-ordered upload of the
+// parameter block (and of the bone matrices changed since the last upload).
+static int upload_params(tfx_hair* h, const tfx_sdf* sdf, hipStream_t stream) {
+    const size_t bone_bytes = (size_t)h->bones_dirty * 16 * sizeof(float);
+    int rc = h->staging.ensure(sizeof(DevParams) + 16 * TFX_MAX_NUM_BONES * sizeof(float));
+    if (rc) return rc;
+    void* host;
+    int slot;
+    if ((rc = h->staging.acquire(&host, &slot))) return rc;
+    DevParams* dp = (DevParams*)host;
+    pack_dev_params(h, sdf, *dp);
+    HIP_TRY(hipMemcpyAsync(h->params_d, dp, sizeof(DevParams), hipMemcpyHostToDevice, stream));
+    if (bone_bytes) {
+        float* hb = (float*)((char*)host + sizeof(DevParams));
+        std::memcpy(hb, h->bones_h.data(), bone_bytes);
+        HIP_TRY(hipMemcpyAsync(h->bones_d, hb, bone_bytes, hipMemcpyHostToDevice, stream));
+        h->bones_dirty = 0;
+    }
+    return h->staging.release_slot(slot, stream);
+}
+
+static DevHair dev_hair(const tfx_hair* h, const tfx_sdf* sdf) {
+    DevHair d;
+    d.pos = h->pos; d.prev = h->prev; d.prevprev = h->prevprev; d.tangent = h->tangent; d.sld = h->sld;
+    d.rest = h->rest; d.restlen = h->restlen; d.follow_off = h->follow_off; d.skin = h->skin;
+    d.bones = h->bones_d; d.params = h->params_d;
+    d.sdf = sdf ? reinterpret_cast<const float*>(sdf->cells) : nullptr;
+    d.G = h->G; d.F = h->F; d.V = h->V; d.pad = 0;
+    return d;
+}
+
+static SdfGrid sdf_grid(const tfx_sdf* s) {
+    SdfGrid g;
+    g.ox = s->origin[0]; g.oy = s->origin[1]; g.oz = s->origin[2];
+    g.cell = s->cell;
+    g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.pad = 0;
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Host math of TressFXHairObject::SetWind (TressFXHairObject.cpp:172-235,
+// Quaternion.cpp:79-89, :262-284; Vector3D.cpp:48-62).
+// ---------------------------------------------------------------------------
+namespace {
+struct Q { float x, y, z, w; };
+struct Vec { float x, y, z; };
+Q q_axis_angle(Vec a, float ang) {
+    float half = 0.5f * ang;
+    float s = std::sin(half);
+    return Q{s * a.x, s * a.y, s * a.z, std::cos(half)};
+}
+Q q_mul(Q a, Q b) {
+    Q q;
+    q.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    q.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    q.y = a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z;
+    q.z = a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x;
+    return q;
+}
+Vec v_cross(Vec a, Vec b) { return Vec{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+Vec q_rot(Q q, Vec v) {
+    Vec qv{q.x, q.y, q.z};
+    Vec uv = v_cross(qv, v), uuv = v_cross(qv, uv);
+    float s = 2.0f * q.w;
+    uv = Vec{uv.x * s, uv.y * s, uv.z * s};
+    uuv = Vec{uuv.x * 2.0f, uuv.y * 2.0f, uuv.z * 2.0f};
+    return Vec{(v.x + uv.x) + uuv.x, (v.y + uv.y) + uuv.y, (v.z + uv.z) + uuv.z};
+}
+Vec v_normalize(Vec v) {
+    float d = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (d == 0) return v;
+    return Vec{v.x / d, v.y / d, v.z / d};
+}
+void set_wind(tfx_simulation_params& cb, const float dir[3], float mag, int frame) {
+    float s = std::sin((float)frame * 0.01f);
+    float wM = mag * (std::pow(s, 2.0f) + 0.5f);
+    Vec wd = v_normalize(Vec{dir[0], dir[1], dir[2]});
+    Vec X{1.0f, 0.0f, 0.0f};
+    Vec xw = v_cross(X, wd);
+    Q rotX{0, 0, 0, 1};
+    float angle = std::asin(std::sqrt(xw.x * xw.x + xw.y * xw.y + xw.z * xw.z));
+    if (angle > 0.001) rotX = q_axis_angle(v_normalize(xw), angle);
+    const float cone = 40.f * 3.14159265359f / 180;
+    const Vec axes[4] = {{0, 1.0f, 0}, {0, -1.0f, 0}, {0, 0, 1.0f}, {0, 0, -1.0f}};
+    float* outs[4] = {cb.wind, cb.wind1, cb.wind2, cb.wind3};
+    for (int k = 0; k < 4; ++k) {
+        Vec nd = q_rot(q_mul(rotX, q_axis_angle(axes[k], cone)), X);
+        outs[k][0] = nd.x * wM; outs[k][1] = nd.y * wM; outs[k][2] = nd.z * wM; outs[k][3] = 0;
+    }
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Library
+// ---------------------------------------------------------------------------
+extern "C" const char* tfx_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int tfx_get_version(int* major, int* minor, int* patch) {
+    if (major) *major = 4;
+    if (minor) *minor = 0;
+    if (patch) *patch = 0;
+    return TFX_OK;
+}
+
+extern "C" int tfx_init(int hip_device) {
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    CHECK_ARG(hip_device >= 0 && hip_device < n, "tfx_init: no such HIP device");
+    HIP_TRY(hipSetDevice(hip_device));
+    g_device = hip_device;
+    return TFX_OK;
+}
+
+extern "C" void tfx_simulation_settings_default(tfx_simulation_settings* s) {
+    if (!s) return;
+    s->vsp_coeff = 0.8f;
+    s->vsp_accel_threshold = 1.4f;
+    s->local_constraint_stiffness = 0.9f;
+    s->local_constraints_iterations = 2;
+    s->global_constraint_stiffness = 0.0f;
+    s->global_constraints_range = 0.0f;
+    s->length_constraints_iterations = 2;
+    s->damping = 0.08f;
+    s->gravity_magnitude = 0.0f;
+    s->tip_separation = 0.0f;
+    s->wind_magnitude = 0.0f;
+    s->wind_direction[0] = 1.0f;
+    s->wind_direction[1] = 0.0f;
+    s->wind_direction[2] = 0.0f;
+    s->wind_angle_radians = 3.1415926f / 180.0f * 40.0f;
+    s->clamp_position_delta = 20.0f;
+}
+
+// ---------------------------------------------------------------------------
+// TressFXHairObject
+// ---------------------------------------------------------------------------
+extern "C" int tfx_hair_create(const tfx_asset_desc* a, tfx_hair** out) {
+    CHECK_ARG(a && out, "tfx_hair_create: null argument");
+    *out = nullptr;
+    const int V = a->num_vertices_per_strand;
+    CHECK_ARG(V > 2 && V <= TFX_SIM_THREAD_GROUP_SIZE && TFX_SIM_THREAD_GROUP_SIZE % V == 0,
+              "tfx_hair_create: vertices per strand must be 4, 8, 16, 32 or 64 (TressFXAsset.cpp:118)");
+    CHECK_ARG(a->num_guide_strands > 0, "tfx_hair_create: no guide strands");
+    CHECK_ARG(a->num_follow_strands_per_guide >= 0, "tfx_hair_create: negative follow count");
+    CHECK_ARG(a->positions && a->rest_lengths && a->bone_skinning_data, "tfx_hair_create: missing asset array");
+    const long long S = (long long)a->num_guide_strands * (a->num_follow_strands_per_guide + 1);
+    const long long N = S * V;
+    CHECK_ARG(N < (1ll << 31), "tfx_hair_create: more than 2^31 vertices");
+    CHECK_ARG(a->num_follow_strands_per_guide == 0 || a->follow_root_offsets,
+              "tfx_hair_create: follow strands need follow_root_offsets");
+    // Bone indices used by the skinning blend must address the 512-matrix table.
+    for (long long s = 0; s < S; s += (a->num_follow_strands_per_guide + 1)) {
+        const tfx_bone_skinning_data& sd = a->bone_skinning_data[s];
+        for (int k = 0; k < 4; ++k) {
+            if (k > 0 && !(sd.bone_weight[k] > 0.0f)) continue;
+            float fi = sd.bone_index[k];
+            if (!(fi >= 0.0f && fi < (float)TFX_MAX_NUM_BONES))
+                return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_hair_create: strand %lld bone index %g outside [0, 512)", s,
+                            (double)fi);
+        }
+    }
+    tfx_hair* h = new tfx_hair();
+    h->G = a->num_guide_strands;
+    h->F = a->num_follow_strands_per_guide;
+    h->V = V;
+    h->S = (int)S;
+    h->N = (int)N;
+    h->device = g_device;
+    set_identity_bones(h->bones_h);
+    h->bones_dirty = TFX_MAX_NUM_BONES;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = dalloc(&h->pos, N);
+    if (e == hipSuccess) e = dalloc(&h->prev, N);
+    if (e == hipSuccess) e = dalloc(&h->prevprev, N);
+    if (e == hipSuccess) e = dalloc(&h->tangent, N);
+    if (e == hipSuccess) e = dalloc(&h->sld, S);
+    if (e == hipSuccess) e = dalloc(&h->rest, N);
+    if (e == hipSuccess) e = dalloc(&h->restlen, N);
+    if (e == hipSuccess) e = dalloc(&h->follow_off, S);
+    if (e == hipSuccess) e = dalloc(&h->skin, S);
+    if (e == hipSuccess) e = dalloc(&h->bones_d, 16 * TFX_MAX_NUM_BONES);
+    if (e == hipSuccess) e = dalloc(&h->params_d, 1);
+    if (e != hipSuccess) {
+        hair_free(h);
+        delete h;
+        return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_hair_create: %s",
+                    hipGetErrorString(e));
+    }
+    // TressFXDynamicState::UploadGPUData: Pos = Prev = PrevPrev = positions.
+    const size_t v4 = (size_t)N * sizeof(float4);
+    std::vector<float> zeros;
+    HIP_TRY(hipMemcpy(h->pos, a->positions, v4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->prev, a->positions, v4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->prevprev, a->positions, v4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->rest, a->positions, v4, hipMemcpyHostToDevice));
+    if (a->tangents) HIP_TRY(hipMemcpy(h->tangent, a->tangents, v4, hipMemcpyHostToDevice));
+    else HIP_TRY(hipMemset(h->tangent, 0, v4));
+    HIP_TRY(hipMemcpy(h->restlen, a->rest_lengths, (size_t)N * sizeof(float), hipMemcpyHostToDevice));
+    if (a->follow_root_offsets)
+        HIP_TRY(hipMemcpy(h->follow_off, a->follow_root_offsets, (size_t)S * sizeof(float4), hipMemcpyHostToDevice));
+    else
+        HIP_TRY(hipMemset(h->follow_off, 0, (size_t)S * sizeof(float4)));
+    HIP_TRY(hipMemcpy(h->skin, a->bone_skinning_data, (size_t)S * sizeof(tfx_bone_skinning_data),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(h->sld, 0, (size_t)S * sizeof(tfx_strand_level_data)));
+    HIP_TRY(hipMemcpy(h->bones_d, h->bones_h.data(), h->bones_h.size() * sizeof(float), hipMemcpyHostToDevice));
+    h->bones_dirty = 0;
+    compute_tangent_w(h, a->tangents);
+    // Constant block before the first UpdateSimulationParameters: zero, with
+    // the counts of the object and the reset flag of the constructor (:157).
+    std::memset(&h->params, 0, sizeof(h->params));
+    h->params.counts[0] = TFX_SIM_THREAD_GROUP_SIZE / V;
+    h->params.counts[1] = h->F;
+    h->params.counts[2] = V;
+    h->params.cpu_local_shape_iterations = 0;
+    h->params.reset_positions = 1.0f;
+    h->params.clamp_position_delta = 0.0f;
+    HIP_TRY(hipDeviceSynchronize());
+    *out = h;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_destroy(tfx_hair* h) {
+    if (!h) return TFX_OK;
+    hipDeviceSynchronize();
+    hair_free(h);
+    delete h;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_update_bone_matrices(tfx_hair* h, const float* m, int n) {
+    CHECK_ARG(h && (m || n == 0) && n >= 0, "tfx_hair_update_bone_matrices: bad argument");
+    n = std::min(n, TFX_MAX_NUM_BONES);
+    std::memcpy(h->bones_h.data(), m, (size_t)n * 16 * sizeof(float));
+    h->bones_dirty = std::max(h->bones_dirty, n);
+    h->params.num_bones = std::max(h->params.num_bones, n);
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_update_simulation_parameters(tfx_hair* h, const tfx_simulation_settings* s, float dt) {
+    CHECK_ARG(h && s, "tfx_hair_update_simulation_parameters: null argument");
+    tfx_simulation_params& cb = h->params;
+    cb.vsp[0] = s->vsp_coeff;
+    cb.vsp[1] = s->vsp_accel_threshold;
+    cb.shape[0] = s->damping;
+    cb.shape[1] = s->local_constraint_stiffness;
+    cb.shape[2] = s->global_constraint_stiffness;
+    cb.shape[3] = s->global_constraints_range;
+    cb.grav_time_tip[0] = s->gravity_magnitude;
+    cb.grav_time_tip[1] = dt;
+    cb.sim_ints[2] = 0;
+    cb.counts[0] = TFX_SIM_THREAD_GROUP_SIZE / h->V;
+    cb.counts[2] = h->V;
+    cb.counts[1] = h->F;
+    cb.grav_time_tip[2] = s->tip_separation;
+    cb.clamp_position_delta = 20.0f;  // TressFXHairObject.cpp:397 (settings value ignored)
+    if (h->V >= TFX_MIN_VERTS_PER_STRAND_FOR_GPU_ITERATION) {
+        cb.sim_ints[1] = s->local_constraints_iterations;
+        cb.cpu_local_shape_iterations = 1;
+    } else {
+        cb.sim_ints[1] = 1;
+        cb.cpu_local_shape_iterations = s->local_constraints_iterations;
+    }
+    cb.sim_ints[0] = s->length_constraints_iterations;
+    set_wind(cb, s->wind_direction, s->wind_magnitude, h->frame);
+    if (h->frame < 2) cb.reset_positions = 1.0f;
+    cb.simulation_frame = h->frame;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_set_collision_capsules(tfx_hair* h, const float* c0, const float* c1, int n) {
+    CHECK_ARG(h && n >= 0 && n <= TFX_MAX_NUM_COLLISION_CAPSULES && (n == 0 || (c0 && c1)),
+              "tfx_hair_set_collision_capsules: bad argument (at most 8 capsules)");
+    std::memset(h->params.center_and_radius0, 0, sizeof(h->params.center_and_radius0));
+    std::memset(h->params.center_and_radius1, 0, sizeof(h->params.center_and_radius1));
+    for (int c = 0; c < n; ++c)
+        for (int k = 0; k < 4; ++k) {
+            h->params.center_and_radius0[c][k] = c0[4 * c + k];
+            h->params.center_and_radius1[c][k] = c1[4 * c + k];
+        }
+    h->params.num_collision_capsules[0] = n;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_reset_positions(tfx_hair* h) {
+    CHECK_ARG(h, "tfx_hair_reset_positions: null hair");
+    h->params.reset_positions = 1.0f;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_increase_simulation_frame(tfx_hair* h) {
+    CHECK_ARG(h, "tfx_hair_increase_simulation_frame: null hair");
+    h->params.reset_positions = 0.0f;
+    h->frame++;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_get_info(const tfx_hair* h, tfx_hair_info* o) {
+    CHECK_ARG(h && o, "tfx_hair_get_info: null argument");
+    o->num_guide_strands = h->G;
+    o->num_follow_strands_per_guide = h->F;
+    o->num_vertices_per_strand = h->V;
+    o->num_total_strands = h->S;
+    o->num_total_vertices = h->N;
+    o->simulation_frame = h->frame;
+    o->cpu_local_shape_iterations = h->params.cpu_local_shape_iterations;
+    o->pad = 0;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_get_simulation_params(const tfx_hair* h, tfx_simulation_params* o, float* bones) {
+    CHECK_ARG(h && o, "tfx_hair_get_simulation_params: null argument");
+    *o = h->params;
+    if (bones) std::memcpy(bones, h->bones_h.data(), h->bones_h.size() * sizeof(float));
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_get_dynamic_state(tfx_hair* h, tfx_dynamic_state* o) {
+    CHECK_ARG(h && o, "tfx_hair_get_dynamic_state: null argument");
+    o->positions = (float*)h->pos;
+    o->positions_prev = (float*)h->prev;
+    o->positions_prev_prev = (float*)h->prevprev;
+    o->tangents = (float*)h->tangent;
+    o->strand_level_data = h->sld;
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_download_state(tfx_hair* h, void* stream, float* pos, float* prev, float* pp, float* tan,
+                                       float* sld) {
+    CHECK_ARG(h, "tfx_hair_download_state: null hair");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t v4 = (size_t)h->N * sizeof(float4);
+    if (pos) HIP_TRY(hipMemcpyAsync(pos, h->pos, v4, hipMemcpyDeviceToHost, s));
+    if (prev) HIP_TRY(hipMemcpyAsync(prev, h->prev, v4, hipMemcpyDeviceToHost, s));
+    if (pp) HIP_TRY(hipMemcpyAsync(pp, h->prevprev, v4, hipMemcpyDeviceToHost, s));
+    if (tan) HIP_TRY(hipMemcpyAsync(tan, h->tangent, v4, hipMemcpyDeviceToHost, s));
+    if (sld) HIP_TRY(hipMemcpyAsync(sld, h->sld, (size_t)h->S * sizeof(tfx_strand_level_data), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return TFX_OK;
+}
+
+extern "C" int tfx_hair_upload_state(tfx_hair* h, void* stream, const float* pos, const float* prev, const float* pp,
+                                     const float* tan, const float* sld) {
+    CHECK_ARG(h, "tfx_hair_upload_state: null hair");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t v4 = (size_t)h->N * sizeof(float4);
+    if (pos) HIP_TRY(hipMemcpyAsync(h->pos, pos, v4, hipMemcpyHostToDevice, s));
+    if (prev) HIP_TRY(hipMemcpyAsync(h->prev, prev, v4, hipMemcpyHostToDevice, s));
+    if (pp) HIP_TRY(hipMemcpyAsync(h->prevprev, pp, v4, hipMemcpyHostToDevice, s));
+    if (tan) HIP_TRY(hipMemcpyAsync(h->tangent, tan, v4, hipMemcpyHostToDevice, s));
+    if (sld) HIP_TRY(hipMemcpyAsync(h->sld, sld, (size_t)h->S * sizeof(tfx_strand_level_data), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (tan) compute_tangent_w(h, tan);
+    return TFX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// TressFXSimulation
+// ---------------------------------------------------------------------------
+extern "C" int tfx_simulation_create(tfx_simulation** out) {
+    CHECK_ARG(out, "tfx_simulation_create: null argument");
+    tfx_simulation* s = new tfx_simulation();
+    s->device = g_device;
+    for (int v = 0; v <= 64; ++v) s->variant[v] = 0;
+    *out = s;
+    return TFX_OK;
+}
+
+extern "C" int tfx_simulation_destroy(tfx_simulation* s) {
+    if (!s) return TFX_OK;
+    hipDeviceSynchronize();
+    hipFree(s->table_d);
+    delete s;
+    return TFX_OK;
+}
+
+extern "C" int tfx_simulation_run_stage(tfx_simulation* sim, void* stream, tfx_hair* h, int stage) {
+    CHECK_ARG(sim && h && stage >= 0 && stage <= 5, "tfx_simulation_run_stage: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = upload_params(h, nullptr, s);
+    if (rc) return rc;
+    DevHair d = dev_hair(h, nullptr);
+    HIP_TRY(tfx_launch_stage(stage, &d, s));
+    return TFX_OK;
+}
+
+// TressFXSimulation::Simulate, TressFXSimulation.cpp:77-112.
+extern "C" int tfx_simulation_simulate(tfx_simulation* sim, void* stream, tfx_hair* const* hairs, int n) {
+    CHECK_ARG(sim && (hairs || n == 0) && n >= 0, "tfx_simulation_simulate: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < n; ++i) {
+        CHECK_ARG(hairs[i], "tfx_simulation_simulate: null hair");
+        int rc = upload_params(hairs[i], nullptr, s);
+        if (rc) return rc;
+    }
+    for (int stage = 0; stage < 6; ++stage) {
+        for (int i = 0; i < n; ++i) {
+            DevHair d = dev_hair(hairs[i], nullptr);
+            const int reps = (stage == 3) ? hairs[i]->params.cpu_local_shape_iterations : 1;
+            for (int r = 0; r < reps; ++r) HIP_TRY(tfx_launch_stage(stage, &d, s));
+        }
+    }
+    for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
+    return TFX_OK;
+}
+
+extern "C" int tfx_simulation_set_launch_shape(tfx_simulation* sim, int V, int variant, int blocks_per_cu) {
+    CHECK_ARG(sim && tfx_fused_block_vertices(V, 0) > 0, "tfx_simulation_set_launch_shape: bad vertices per strand");
+    CHECK_ARG(variant == 0 || variant == 1, "tfx_simulation_set_launch_shape: variant must be 0 or 1");
+    CHECK_ARG(blocks_per_cu >= 0 && blocks_per_cu <= 64, "tfx_simulation_set_launch_shape: blocks_per_cu out of range");
+    sim->variant[V] = variant;
+    if (blocks_per_cu > 0) sim->grid_per_cu = blocks_per_cu;
+    return TFX_OK;
+}
+
+static int ensure_table(tfx_simulation* sim, size_t bytes) {
+    if (bytes <= sim->table_bytes) return TFX_OK;
+    if (sim->table_d) { HIP_TRY(hipDeviceSynchronize()); HIP_TRY(hipFree(sim->table_d)); sim->table_d = nullptr; }
+    size_t cap = std::max<size_t>(bytes * 2, 1 << 16);
+    HIP_TRY(hipMalloc(&sim->table_d, cap));
+    sim->table_bytes = cap;
+    return TFX_OK;
+}
+
+static int cu_count(int device) {
+    static int cached[64] = {0};
+    if (device >= 0 && device < 64 && cached[device]) return cached[device];
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    if (device >= 0 && device < 64) cached[device] = n;
+    return n;
+}
+
+// One fused substep for a set of objects: per V class, one persistent launch
+// over the (object, strand block) work list.
+extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, tfx_hair* const* hairs,
+                                             tfx_sdf* const* sdfs, int n) {
+    CHECK_ARG(sim && (hairs || n == 0) && n >= 0, "tfx_simulation_simulate_fused: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < n; ++i) {
+        CHECK_ARG(hairs[i], "tfx_simulation_simulate_fused: null hair");
+        int rc = upload_params(hairs[i], sdfs ? sdfs[i] : nullptr, s);
+        if (rc) return rc;
+    }
+    // Table layout: DevHair[n] | WorkItem[total]
+    std::vector<tfx::WorkItem> items;
+    std::vector<int> vclass_begin(65, 0), vclass_count(65, 0);
+    for (int V = 4; V <= 64; V *= 2) {
+        vclass_begin[V] = (int)items.size();
+        const int B = tfx_fused_block_vertices(V, sim->variant[V]) / V;
+        for (int i = 0; i < n; ++i)
+            if (hairs[i]->V == V)
+                for (int g0 = 0; g0 < hairs[i]->G; g0 += B) items.push_back(tfx::WorkItem{i, g0});
+        vclass_count[V] = (int)items.size() - vclass_begin[V];
+    }
+    const size_t hair_bytes = (size_t)n * sizeof(DevHair);
+    const size_t item_off = (hair_bytes + 255) & ~size_t(255);
+    const size_t bytes = item_off + items.size() * sizeof(tfx::WorkItem);
+    int rc = ensure_table(sim, bytes);
+    if (rc) return rc;
+    if ((rc = sim->staging.ensure(bytes))) return rc;
+    void* host;
+    int slot;
+    if ((rc = sim->staging.acquire(&host, &slot))) return rc;
+    DevHair* dh = (DevHair*)host;
+    for (int i = 0; i < n; ++i) dh[i] = dev_hair(hairs[i], sdfs ? sdfs[i] : nullptr);
+    std::memcpy((char*)host + item_off, items.data(), items.size() * sizeof(tfx::WorkItem));
+    HIP_TRY(hipMemcpyAsync(sim->table_d, host, bytes, hipMemcpyHostToDevice, s));
+    if ((rc = sim->staging.release_slot(slot, s))) return rc;
+    const DevHair* dtab = (const DevHair*)sim->table_d;
+    const tfx::WorkItem* ditems = (const tfx::WorkItem*)((char*)sim->table_d + item_off);
+    const int cus = cu_count(sim->device);
+    for (int V = 4; V <= 64; V *= 2) {
+        const int cnt = vclass_count[V];
+        if (!cnt) continue;
+        const int grid = std::min(cnt, cus * sim->grid_per_cu);
+        HIP_TRY(tfx_launch_fused(V, sim->variant[V], dtab, ditems + vclass_begin[V], cnt, grid, s));
+    }
+    for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
+    return TFX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// TressFXSDFCollisionSystem / TressFXSDFCollision
+// ---------------------------------------------------------------------------
+extern "C" int tfx_sdf_collision_system_create(tfx_sdf_collision_system** out) {
+    CHECK_ARG(out, "tfx_sdf_collision_system_create: null argument");
+    *out = new tfx_sdf_collision_system();
+    (*out)->device = g_device;
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_collision_system_destroy(tfx_sdf_collision_system* s) {
+    delete s;
+    return TFX_OK;
+}
+
+static int sdf_alloc(tfx_sdf* s) {
+    const long long cells = (long long)s->nx * s->ny * s->nz;
+    CHECK_ARG(s->nx > 0 && s->ny > 0 && s->nz > 0 && cells < (1ll << 31), "tfx_sdf: grid dimensions out of range");
+    CHECK_ARG(s->total_cells >= cells, "tfx_sdf: allocation smaller than the grid");
+    hipError_t e = hipMalloc((void**)&s->cells, (size_t)s->total_cells * sizeof(uint32_t));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_sdf: %s", hipGetErrorString(e));
+    HIP_TRY(hipMemset(s->cells, 0, (size_t)s->total_cells * sizeof(uint32_t)));
+    return TFX_OK;
+}
+
+// TressFXSDFCollision constructor, TressFXSDFCollision.cpp:38-86.
+extern "C" int tfx_sdf_create(const tfx_sdf_mesh_desc* mesh, int numCellsInX, float margin, tfx_sdf** out) {
+    CHECK_ARG(mesh && out && numCellsInX > 0, "tfx_sdf_create: bad argument");
+    *out = nullptr;
+    tfx_sdf* s = new tfx_sdf();
+    s->mesh = *mesh;
+    s->margin = margin;
+    s->device = g_device;
+    const float* bmin0 = mesh->initial_bbox_min;
+    const float* bmax0 = mesh->initial_bbox_max;
+    s->cell = (bmax0[0] - bmin0[0]) / numCellsInX;
+    const int extra = (int)(0.8f * (float)numCellsInX);
+    // m_PaddingBoundary = a, b, c; — a comma expression: every axis gets `a`.
+    s->pad_boundary = extra * s->cell;
+    float bmin[3], bmax[3];
+    for (int k = 0; k < 3; ++k) {
+        s->origin[k] = bmin0[k] - s->pad_boundary;  // UpdateSDFGrid(initial bbox)
+        bmin[k] = bmin0[k] - s->pad_boundary;
+        bmax[k] = bmax0[k] + s->pad_boundary;
+    }
+    s->nx = (int)((bmax[0] - bmin[0]) / s->cell);
+    s->ny = (int)((bmax[1] - bmin[1]) / s->cell);
+    s->nz = (int)((bmax[2] - bmin[2]) / s->cell);
+    const float alloc = 1.4f * s->nx * s->ny * s->nz;
+    s->total_cells = (int)std::min<double>((double)alloc, 2147483647.0);
+    int rc = sdf_alloc(s);
+    if (rc) { delete s; return rc; }
+    *out = s;
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_create_explicit(const tfx_sdf_mesh_desc* mesh, const float origin[3], float cell, int nx,
+                                       int ny, int nz, float margin, tfx_sdf** out) {
+    CHECK_ARG(mesh && origin && out && cell > 0.0f, "tfx_sdf_create_explicit: bad argument");
+    *out = nullptr;
+    tfx_sdf* s = new tfx_sdf();
+    s->mesh = *mesh;
+    s->margin = margin;
+    s->device = g_device;
+    s->explicit_grid = 1;
+    s->cell = cell;
+    for (int k = 0; k < 3; ++k) s->origin[k] = origin[k];
+    s->nx = nx; s->ny = ny; s->nz = nz;
+    s->total_cells = (int)((long long)nx * ny * nz);
+    int rc = sdf_alloc(s);
+    if (rc) { delete s; return rc; }
+    *out = s;
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_destroy(tfx_sdf* s) {
+    if (!s) return TFX_OK;
+    hipDeviceSynchronize();
+    hipFree(s->cells);
+    delete s;
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_set_mesh(tfx_sdf* s, const tfx_sdf_mesh_desc* mesh) {
+    CHECK_ARG(s && mesh, "tfx_sdf_set_mesh: null argument");
+    s->mesh = *mesh;
+    return TFX_OK;
+}
+
+// TressFXSDFCollision::Update, TressFXSDFCollision.cpp:95-169.
+extern "C" int tfx_sdf_update(tfx_sdf* s, tfx_sdf_collision_system* sys, void* stream) {
+    CHECK_ARG(s && sys, "tfx_sdf_update: null argument");
+    if (!s->mesh.vertices || !s->mesh.indices) return TFX_OK;  // no input mesh: nothing to do (:100)
+    if (!s->explicit_grid)
+        for (int k = 0; k < 3; ++k) s->origin[k] = s->mesh.bbox_min[k] - s->pad_boundary;  // UpdateSDFGrid
+    const int ncell = s->nx * s->ny * s->nz;
+    HIP_TRY(tfx_launch_sdf_build(s->cells, ncell, s->mesh.vertices, s->mesh.indices, s->mesh.num_triangles,
+                                 sdf_grid(s), (hipStream_t)stream));
+    return TFX_OK;
+}
+
+// TressFXSDFCollision::CollideWithHair, TressFXSDFCollision.cpp:171-221.
+extern "C" int tfx_sdf_collide_with_hair(tfx_sdf* s, tfx_sdf_collision_system* sys, void* stream, tfx_hair* h) {
+    CHECK_ARG(s && sys && h, "tfx_sdf_collide_with_hair: null argument");
+    if (!s->mesh.vertices || !s->mesh.indices) return TFX_OK;
+    HIP_TRY(tfx_launch_sdf_collide(h->pos, h->prev, h->N, h->V, reinterpret_cast<const float*>(s->cells), sdf_grid(s),
+                                   s->margin * s->cell, (hipStream_t)stream));
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_get_grid(const tfx_sdf* s, tfx_sdf_grid_info* o) {
+    CHECK_ARG(s && o, "tfx_sdf_get_grid: null argument");
+    for (int k = 0; k < 3; ++k) o->origin[k] = s->origin[k];
+    o->cell_size = s->cell;
+    o->num_cells_x = s->nx; o->num_cells_y = s->ny; o->num_cells_z = s->nz;
+    o->num_total_cells = s->total_cells;
+    o->collision_margin = s->margin;
+    o->padding_boundary = s->pad_boundary;
+    o->explicit_grid = s->explicit_grid;
+    o->pad = 0;
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_get_data_buffer(tfx_sdf* s, float** p) {
+    CHECK_ARG(s && p, "tfx_sdf_get_data_buffer: null argument");
+    *p = reinterpret_cast<float*>(s->cells);
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_download(tfx_sdf* s, void* stream, float* host) {
+    CHECK_ARG(s && host, "tfx_sdf_download: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(host, s->cells, (size_t)s->nx * s->ny * s->nz * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return TFX_OK;
+}
+
+extern "C" int tfx_sdf_get_collision_params(const tfx_sdf* s, const tfx_hair* h, tfx_sdf_collision_params* o) {
+    CHECK_ARG(s && h && o, "tfx_sdf_get_collision_params: null argument");
+    std::memset(o, 0, sizeof(*o));
+    for (int k = 0; k < 3; ++k) o->origin[k] = s->origin[k];
+    o->cell_size = s->cell;
+    o->num_cells_x = s->nx; o->num_cells_y = s->ny; o->num_cells_z = s->nz;
+    o->collision_margin = s->margin * s->cell;
+    o->num_total_hair_vertices = h->N;
+    o->num_hair_vertices_per_strand = h->V;
+    return TFX_OK;
+}

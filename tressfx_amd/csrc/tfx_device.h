@@ -1,0 +1,344 @@
+// tfx_device.h — device-side records and FP32 math for the gfx950 kernels.
+//
+// The arithmetic helpers fix the operation order that HLSL leaves open, so
+// that a kernel evaluates each reference expression in one defined way:
+//   dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   normalize(v) = v / sqrt(dot(v,v)) (IEEE division and square root)
+//   min/max = IEEE minNum/maxNum (v_min_f32 / v_max_f32)
+//   float -> int = truncation, NaN -> 0, saturating (D3D ftoi)
+// All HIP sources are compiled with -ffp-contract=off, so no product is
+// fused into an add behind the source's back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tressfx_amd.h"
+
+namespace tfx {
+
+struct f3 { float x, y, z; };
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 xyz(const float4& a) { return f3{a.x, a.y, a.z}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 mul(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 mul(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 divs(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 normalize3(f3 a) { return divs(a, sqrtf(dot3(a, a))); }
+__device__ __forceinline__ float fmn(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float fmx(float a, float b) { return fmaxf(a, b); }
+
+__device__ __forceinline__ int ftoi(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)x;
+}
+
+__device__ __forceinline__ bool movable(const float4& p) { return p.w > 0.0f; }
+
+// MakeQuaternion(float4x4), TressFXSimulation.hlsl:247-288, on the blended
+// matrix's upper 3x3 (m[r*3+c] = logical element (r, c)).
+__device__ __forceinline__ float4 quat_from_matrix3(const float m[9]) {
+    float4 q;
+    float trace = m[0] + m[4] + m[8];
+    if (trace > 0.0f) {
+        float r = sqrtf(trace + 1.0f);
+        q.w = 0.5f * r;
+        r = 0.5f / r;
+        q.x = (m[5] - m[7]) * r;
+        q.y = (m[6] - m[2]) * r;
+        q.z = (m[1] - m[3]) * r;
+    } else {
+        // (i, j, k) is one of the cyclic triples (0,1,2), (1,2,0), (2,0,1);
+        // each is written out so no array is indexed at run time.
+        int i = 0;
+        if (m[4] > m[0]) i = 1;
+        if (m[8] > m[i * 4]) i = 2;
+        if (i == 0) {
+            float r = sqrtf(m[0] - m[4] - m[8] + 1.0f);
+            q.x = 0.5f * r;
+            r = 0.5f / r;
+            q.w = (m[5] - m[7]) * r;
+            q.y = (m[3] + m[1]) * r;
+            q.z = (m[6] + m[2]) * r;
+        } else if (i == 1) {
+            float r = sqrtf(m[4] - m[8] - m[0] + 1.0f);
+            q.y = 0.5f * r;
+            r = 0.5f / r;
+            q.w = (m[6] - m[2]) * r;
+            q.z = (m[7] + m[5]) * r;
+            q.x = (m[1] + m[3]) * r;
+        } else {
+            float r = sqrtf(m[8] - m[0] - m[4] + 1.0f);
+            q.z = 0.5f * r;
+            r = 0.5f / r;
+            q.w = (m[1] - m[3]) * r;
+            q.x = (m[2] + m[6]) * r;
+            q.y = (m[5] + m[7]) * r;
+        }
+    }
+    return q;
+}
+
+// MultQuaternionAndVector :305-315
+__device__ __forceinline__ f3 qrot(const float4& q, f3 v) {
+    f3 qv{q.x, q.y, q.z};
+    f3 uv = cross3(qv, v);
+    f3 uuv = cross3(qv, uv);
+    uv = mul(uv, 2.0f * q.w);
+    uuv = mul(uuv, 2.0f);
+    return add(add(v, uv), uuv);
+}
+
+// NormalizeQuaternion :329-342
+__device__ __forceinline__ float4 normalize_quat(float4 q) {
+    float n = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+    if (n < 1e-10f) { q.w = 1.0f; return q; }
+    float s = 1.0f / sqrtf(n);
+    return make_float4(q.x * s, q.y * s, q.z * s, q.w * s);
+}
+
+// QuatFromTwoUnitVectors :345-363
+__device__ __forceinline__ float4 quat_two_vectors(f3 u, f3 v) {
+    float r = 1.0f + dot3(u, v);
+    f3 n;
+    if (r < 1e-7f) {
+        r = 0.0f;
+        n = fabsf(u.x) > fabsf(u.z) ? f3{-u.y, u.x, 0.0f} : f3{0.0f, -u.z, u.y};
+    } else {
+        n = cross3(u, v);
+    }
+    return normalize_quat(make_float4(n.x, n.y, n.z, r));
+}
+
+// Blended bone of one strand: ApplyVertexBoneSkinning's matrix path
+// (:549-567) — 3x3 rotation/scale part plus translation row.
+struct BlendedBone {
+    float m[9];   // rows 0..2, cols 0..2
+    float t[3];   // row 3, cols 0..2
+};
+
+__device__ __forceinline__ const float* bone_ptr(const float* bones, float fidx) {
+    int idx = ftoi(fidx);
+    idx = max(0, min(idx, TFX_MAX_NUM_BONES - 1));
+    return bones + 16 * idx;
+}
+
+// Blends the 4x4 matrices element-wise (the unused fourth column is skipped:
+// no result of the path reads it).
+__device__ __forceinline__ void blend_bones(const tfx_bone_skinning_data& sd, const float* bones, BlendedBone& b) {
+    const float* m0 = bone_ptr(bones, sd.bone_index[0]);
+    const float w0 = sd.bone_weight[0];
+    float acc[12];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[r * 3 + c] = m0[4 * r + c] * w0;
+    float wsum = w0;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+        const float wi = sd.bone_weight[i];
+        if (wi > 0.0f) {
+            const float* mi = bone_ptr(bones, sd.bone_index[i]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[r * 3 + c] = acc[r * 3 + c] + mi[4 * r + c] * wi;
+            wsum = wsum + wi;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b.m[k] = acc[k] / wsum;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) b.t[k] = acc[9 + k] / wsum;
+}
+
+// mul(float4(p, 1), M).xyz
+__device__ __forceinline__ f3 bone_transform(const float m[9], const float t[3], f3 p) {
+    return f3{((p.x * m[0] + p.y * m[3]) + p.z * m[6]) + t[0],
+              ((p.x * m[1] + p.y * m[4]) + p.z * m[7]) + t[1],
+              ((p.x * m[2] + p.y * m[5]) + p.z * m[8]) + t[2]};
+}
+
+// ApplyDistanceConstraint :377-387 with ConstraintMultiplier :214-230.
+__device__ __forceinline__ void distance_constraint(float4& p0, float4& p1, float target) {
+    f3 delta = sub(xyz(p1), xyz(p0));
+    float distance = fmx(len3(delta), 1e-7f);
+    float stretching = 1.0f - target / distance;
+    delta = mul(stretching, delta);
+    const bool a = movable(p0), b = movable(p1);
+    const float m0 = a ? (b ? 0.5f : 1.0f) : 0.0f;
+    const float m1 = a ? (b ? 0.5f : 0.0f) : (b ? 1.0f : 0.0f);
+    f3 d0 = mul(m0, delta);
+    f3 d1 = mul(m1, delta);
+    p0.x += d0.x; p0.y += d0.y; p0.z += d0.z;
+    p1.x -= d1.x; p1.y -= d1.y; p1.z -= d1.z;
+}
+
+// CapsuleCollision :462-522 (friction 0.4).
+__device__ __forceinline__ bool capsule_collision(const float4& cur, const float4& old, f3& newp,
+                                                  const float4& c0, const float4& c1) {
+    const float friction = 0.4f;
+    const float radius0 = c0.w, radius1 = c1.w;
+    newp = xyz(cur);
+    if (!movable(cur)) return false;
+    f3 segment = sub(xyz(c1), xyz(c0));
+    f3 delta0 = sub(xyz(cur), xyz(c0));
+    f3 delta1 = sub(xyz(c1), xyz(cur));
+    float dist0 = dot3(delta0, segment);
+    float dist1 = dot3(delta1, segment);
+    if (dist0 < 0.0f) {
+        if (dot3(delta0, delta0) < radius0 * radius0) {
+            f3 n = normalize3(delta0);
+            newp = add(mul(radius0, n), xyz(c0));
+            return true;
+        }
+        return false;
+    }
+    if (dist1 < 0.0f) {
+        if (dot3(delta1, delta1) < radius1 * radius1) {
+            f3 n = normalize3(f3{-delta1.x, -delta1.y, -delta1.z});
+            newp = add(mul(radius1, n), xyz(c1));
+            return true;
+        }
+        return false;
+    }
+    f3 x = divs(add(mul(dist0, xyz(c1)), mul(dist1, xyz(c0))), dist0 + dist1);
+    f3 delta = sub(xyz(cur), x);
+    float radius_at_x = (dist0 * radius1 + dist1 * radius0) / (dist0 + dist1);
+    if (dot3(delta, delta) < radius_at_x * radius_at_x) {
+        f3 n = normalize3(delta);
+        f3 vec = sub(xyz(cur), xyz(old));
+        f3 segN = normalize3(segment);
+        f3 vecTangent = mul(dot3(vec, segN), segN);
+        f3 vecNormal = sub(vec, vecTangent);
+        newp = add(add(xyz(old), mul(friction, vecTangent)), sub(add(vecNormal, mul(radius_at_x, n)), delta));
+        return true;
+    }
+    return false;
+}
+
+// Per-object, per-substep parameter block as the kernels read it: the
+// tfx_simulation_params fields the path uses, the host-evaluated decay
+// exp(-damping*dt*60) and the object's SDF collision constants.
+struct alignas(16) DevParams {
+    float4 wind0, wind1, wind2, wind3;
+    float damping, local_stiffness, global_stiffness, global_range;
+    float gravity, dt, tip_sep, decay;
+    int length_iters, lsc_passes, wind_on, num_capsules;
+    float vsp_coeff, vsp_accel_threshold, reset, clamp;
+    float4 cap0[TFX_MAX_NUM_COLLISION_CAPSULES];
+    float4 cap1[TFX_MAX_NUM_COLLISION_CAPSULES];
+    // SDF collision for the fused substep (CollideHairVerticesWithSdf_forward)
+    float sdf_ox, sdf_oy, sdf_oz, sdf_cell;
+    int sdf_nx, sdf_ny, sdf_nz, has_sdf;
+    float sdf_margin;
+    int tangent_w_uniform;  // all guide tangent w equal -> tangent_w
+    float tangent_w;
+    int pad;
+};
+
+// Device pointers and counts of one hair object.
+struct DevHair {
+    float4* pos;
+    float4* prev;
+    float4* prevprev;
+    float4* tangent;
+    tfx_strand_level_data* sld;
+    const float4* rest;
+    const float* restlen;
+    const float4* follow_off;
+    const tfx_bone_skinning_data* skin;
+    const float* bones;          // 512 x 16
+    const DevParams* params;
+    const float* sdf;            // float cells of the hair's SDF (fused path)
+    int G, F, V, pad;
+};
+
+// One entry of the fused kernel's work list: a block of guide strands of one
+// object.
+struct WorkItem {
+    int hair;   // index into the DevHair table
+    int g0;     // first guide strand of the block
+};
+
+// SDF grid constants (TressFXSDFCollisionParams subset).
+struct SdfGrid {
+    float ox, oy, oz, cell;
+    int nx, ny, nz, pad;
+};
+
+// GetSdfCoordinates :103-113
+__device__ __forceinline__ void sdf_coords(const SdfGrid& g, f3 p, int& cx, int& cy, int& cz) {
+    cx = ftoi((p.x - g.ox) / g.cell);
+    cy = ftoi((p.y - g.oy) / g.cell);
+    cz = ftoi((p.z - g.oz) / g.cell);
+}
+// GetSdfCellIndex :123-129 (wrapping int arithmetic)
+__device__ __forceinline__ int sdf_index(const SdfGrid& g, int x, int y, int z) {
+    uint32_t perLine = (uint32_t)g.nx, perPlane = (uint32_t)g.nx * (uint32_t)g.ny;
+    return (int)(perPlane * (uint32_t)z + perLine * (uint32_t)y + (uint32_t)x);
+}
+// GetSdfCellPosition :115-121
+__device__ __forceinline__ f3 sdf_cell_pos(const SdfGrid& g, int x, int y, int z) {
+    return f3{(float)x * g.cell + g.ox, (float)y * g.cell + g.oy, (float)z * g.cell + g.oz};
+}
+
+__device__ __forceinline__ float lerp_(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// GetSignedDistance :472-527 (1e10 = outside the band or the grid)
+__device__ __forceinline__ float sdf_distance(const SdfGrid& g, const float* __restrict__ sdf, f3 p) {
+    const float INITIAL = 1e10f;
+    int cx, cy, cz;
+    sdf_coords(g, p, cx, cy, cz);
+    if (!(0 <= cx && cx < g.nx - 2) || !(0 <= cy && cy < g.ny - 2) || !(0 <= cz && cz < g.nz - 2)) return INITIAL;
+    const int base = sdf_index(g, cx, cy, cz);
+    const int X = 1, Y = g.nx, Z = g.ny * g.nx;
+    const int ids[8] = {base, base + X, base + Y, base + Y + X, base + Z, base + Z + X, base + Z + Y, base + Z + Y + X};
+    float d[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float v = sdf[ids[j]];
+        if (v == INITIAL) return INITIAL;
+        d[j] = v;
+    }
+    f3 cp = sdf_cell_pos(g, cx, cy, cz);
+    f3 t = divs(sub(p, cp), g.cell);
+    float a = lerp_(lerp_(d[0], d[1], t.x), lerp_(d[2], d[3], t.x), t.y);
+    float b = lerp_(lerp_(d[4], d[5], t.x), lerp_(d[6], d[7], t.x), t.y);
+    return lerp_(a, b, t.z);
+}
+
+// CollideHairVerticesWithSdf_forward :546-595 for one vertex position.
+// Returns true (and the projected position) when the vertex is moved.
+__device__ __forceinline__ bool sdf_collide_vertex(const SdfGrid& g, const float* __restrict__ sdf, float margin,
+                                                   f3 p, f3& out) {
+    float distance = sdf_distance(g, sdf, p);
+    if (distance > margin) return false;
+    const float h = 0.1f * g.cell;
+    float fx = sdf_distance(g, sdf, f3{p.x + 1.0f * h, p.y + 0.0f * h, p.z + 0.0f * h});
+    float fy = sdf_distance(g, sdf, f3{p.x + 0.0f * h, p.y + 1.0f * h, p.z + 0.0f * h});
+    float fz = sdf_distance(g, sdf, f3{p.x + 0.0f * h, p.y + 0.0f * h, p.z + 1.0f * h});
+    f3 grad = divs(f3{fx - distance, fy - distance, fz - distance}, h);
+    f3 n = normalize3(grad);
+    if (distance < margin) {
+        out = add(p, mul(n, margin - distance));
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void store_xyz(float4* p, f3 v) {
+    float* f = reinterpret_cast<float*>(p);
+    f[0] = v.x; f[1] = v.y; f[2] = v.z;
+}
+
+}  // namespace tfx

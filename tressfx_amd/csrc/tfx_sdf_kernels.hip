@@ -1,0 +1,177 @@
+// tfx_sdf_kernels.hip — signed distance field build from a triangle mesh and
+// hair-vs-SDF collision, TressFXSDFCollision.hlsl.
+//
+// Build: Initialize (fill FloatFlip3(1e10)) -> Construct (per triangle, an
+// atomicMin of FloatFlip3(signed distance) into every node of the triangle's
+// padded cell box) -> Finalize (IFloatFlip3 in place).  atomicMin on the
+// rotated-sign bit pattern orders by |d| with ties going to +d, so the grid is
+// bit-identical for any triangle order or lane mapping.
+//
+// Construct maps a team of lanes to one triangle and spreads the triangle's
+// cell box over the team (instead of one thread looping over the whole box), so
+// large and small triangles share a wave without one lane serialising it.
+#include "tfx_device.h"
+
+namespace tfx {
+
+constexpr int kSdfBlock = 256;
+
+// FloatFlip3 / IFloatFlip3 :92-100
+__device__ __forceinline__ uint32_t float_flip3(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u << 1) | (u >> 31);
+}
+__device__ __forceinline__ uint32_t ifloat_flip3(uint32_t u) { return (u >> 1) | (u << 31); }
+
+// DistancePointToEdge :147-159 (the normal it also forms is unused)
+__device__ __forceinline__ float distance_point_to_edge(f3 p, f3 x0, f3 x1) {
+    f3 x10 = sub(x1, x0);
+    float t = dot3(sub(x1, p), x10) / dot3(x10, x10);
+    t = fmx(0.0f, fmn(t, 1.0f));
+    f3 a = sub(p, add(mul(t, x0), mul(1.0f - t, x1)));
+    return len3(a);
+}
+
+// Per-triangle constants of SignedDistancePointToTriangle :163-215 that do
+// not depend on the query point.
+struct TriPrep {
+    f3 x0, x1, x2;
+    f3 x02n;   // normalised x0 - x2
+    f3 x12n;   // x1 - x2 made orthogonal to x02n, normalised
+    float l0, l1, l2;
+    f3 nTri;
+};
+
+__device__ __forceinline__ TriPrep tri_prep(f3 x0, f3 x1, f3 x2) {
+    TriPrep t;
+    t.x0 = x0; t.x1 = x1; t.x2 = x2;
+    f3 x02 = sub(x0, x2);
+    t.l0 = len3(x02) + 1e-30f;
+    x02 = divs(x02, t.l0);
+    f3 x12 = sub(x1, x2);
+    t.l1 = dot3(x12, x02);
+    x12 = sub(x12, mul(t.l1, x02));
+    t.l2 = len3(x12) + 1e-30f;
+    x12 = divs(x12, t.l2);
+    t.x02n = x02;
+    t.x12n = x12;
+    t.nTri = cross3(sub(x1, x0), sub(x2, x0));
+    return t;
+}
+
+__device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p) {
+    f3 px2 = sub(p, t.x2);
+    float b = dot3(t.x12n, px2) / t.l2;
+    float a = (dot3(t.x02n, px2) - t.l1 * b) / t.l0;
+    float c = (1.0f - a) - b;
+    const float tol = 1e-8f;
+    float d;
+    if (a >= -tol && b >= -tol && c >= -tol) {
+        f3 n = sub(p, add(add(mul(a, t.x0), mul(b, t.x1)), mul(c, t.x2)));
+        d = len3(n);
+    } else {
+        d = distance_point_to_edge(p, t.x0, t.x1);
+        float d12 = distance_point_to_edge(p, t.x1, t.x2);
+        float d02 = distance_point_to_edge(p, t.x0, t.x2);
+        d = fmn(d, d12);
+        d = fmn(d, d02);
+    }
+    return (dot3(sub(p, t.x0), t.nTri) < 0.0f) ? -d : d;
+}
+
+// InitializeSignedDistanceField :301-312
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_init(uint32_t* __restrict__ sdf, int ncell) {
+    const uint32_t v = float_flip3(1e10f);
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock) sdf[i] = v;
+}
+
+// FinalizeSignedDistanceField :400-412
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize(uint32_t* __restrict__ sdf, int ncell) {
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock)
+        sdf[i] = ifloat_flip3(sdf[i]);
+}
+
+// ConstructSignedDistanceField :329-397.  kTeam lanes per triangle; the team
+// walks the triangle's node box in x-fastest order.
+template <int kTeam>
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_construct(uint32_t* __restrict__ sdf,
+                                                             const tfx_standard_vertex* __restrict__ verts,
+                                                             const uint32_t* __restrict__ idx, int ntri, SdfGrid g) {
+    const int lane = threadIdx.x % kTeam;
+    const int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam;
+    if (tri >= ntri) return;
+    const uint32_t i0 = idx[3 * tri + 0], i1 = idx[3 * tri + 1], i2 = idx[3 * tri + 2];
+    const f3 x0{verts[i0].position[0], verts[i0].position[1], verts[i0].position[2]};
+    const f3 x1{verts[i1].position[0], verts[i1].position[1], verts[i1].position[2]};
+    const f3 x2{verts[i2].position[0], verts[i2].position[1], verts[i2].position[2]};
+    const float m = g.cell;  // MARGIN
+    f3 mn{fmn(x0.x, fmn(x1.x, x2.x)) - m, fmn(x0.y, fmn(x1.y, x2.y)) - m, fmn(x0.z, fmn(x1.z, x2.z)) - m};
+    f3 mx{fmx(x0.x, fmx(x1.x, x2.x)) + m, fmx(x0.y, fmx(x1.y, x2.y)) + m, fmx(x0.z, fmx(x1.z, x2.z)) + m};
+    int lo[3], hi[3];
+    sdf_coords(g, mn, lo[0], lo[1], lo[2]);
+    sdf_coords(g, mx, hi[0], hi[1], hi[2]);
+    const int dims[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = (int)((uint32_t)lo[k] - 1u);
+        hi[k] = (int)((uint32_t)hi[k] + 1u);
+        lo[k] = max(0, min(lo[k], dims[k] - 1));
+        hi[k] = max(0, min(hi[k], dims[k] - 1));
+    }
+    const TriPrep tp = tri_prep(x0, x1, x2);
+    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    const int total = ex * ey * ez;
+    for (int c = lane; c < total; c += kTeam) {
+        const int xx = c % ex;
+        const int rem = c / ex;
+        const int yy = rem % ey;
+        const int zz = rem / ey;
+        const int x = lo[0] + xx, y = lo[1] + yy, z = lo[2] + zz;
+        const f3 cp = sdf_cell_pos(g, x, y, z);
+        const float d = signed_distance(tp, cp);
+        atomicMin(&sdf[sdf_index(g, x, y, z)], float_flip3(d));
+    }
+}
+
+// CollideHairVerticesWithSdf_forward :531-595 — one lane per vertex, guide
+// and follow strands alike; local vertices 0 and 1 are skipped.
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_collide(float4* __restrict__ pos, float4* __restrict__ prev,
+                                                           int n, int V, const float* __restrict__ sdf, SdfGrid g,
+                                                           float margin) {
+    const int i = blockIdx.x * kSdfBlock + threadIdx.x;
+    if (i >= n) return;
+    const int lv = i % V;
+    if (lv == 0 || lv == 1) return;
+    const float4 hv = pos[i];
+    f3 out;
+    if (sdf_collide_vertex(g, sdf, margin, xyz(hv), out)) {
+        store_xyz(&pos[i], out);
+        store_xyz(&prev[i], out);
+    }
+}
+
+}  // namespace tfx
+
+extern "C" hipError_t tfx_launch_sdf_build(uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
+                                           const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream) {
+    using namespace tfx;
+    const int cblocks = std::min((ncell + kSdfBlock - 1) / kSdfBlock, 4096);
+    if (ncell > 0) hipLaunchKernelGGL(k_sdf_init, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
+    if (ntri > 0) {
+        constexpr int kTeam = 16;
+        const long long lanes = (long long)ntri * kTeam;
+        const int tblocks = (int)((lanes + kSdfBlock - 1) / kSdfBlock);
+        hipLaunchKernelGGL(k_sdf_construct<kTeam>, dim3(tblocks), dim3(kSdfBlock), 0, stream, sdf, verts, idx, ntri, g);
+    }
+    if (ncell > 0) hipLaunchKernelGGL(k_sdf_finalize, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, int V, const float* sdf,
+                                             tfx::SdfGrid g, float margin, hipStream_t stream) {
+    using namespace tfx;
+    if (n <= 0) return hipSuccess;
+    const int blocks = (n + kSdfBlock - 1) / kSdfBlock;
+    hipLaunchKernelGGL(k_sdf_collide, dim3(blocks), dim3(kSdfBlock), 0, stream, pos, prev, n, V, sdf, g, margin);
+    return hipGetLastError();
+}

@@ -1,0 +1,314 @@
+"""Seeded synthetic workloads (hair assets, collision meshes, trajectories).
+
+Seeds follow `seed = 1000 * config + instance`; the generator is numpy's
+PCG64.  Conventions of the reference's assets are kept:
+  - positions are float4 with w = inverse mass: 0 for vertices 0 and 1, 1 after
+    (tool/Maya/TressFX_Exporter.py:1585-1592);
+  - rest lengths are |p_j - p_{j+1}| with a trailing 0 per strand
+    (TressFXAsset.cpp:377-399);
+  - tangents follow ComputeStrandTangent (TressFXAsset.cpp:304-342); their w,
+    which the reference leaves uninitialised, is 0 here;
+  - follow strands are interleaved after their guide, offset in the root's
+    tangent plane, w copied from the guide, offset.w = guide strand index
+    (TressFXAsset.cpp:177-248);
+  - bone matrices are row-major row-vector 4x4 (p' = [p,1] M).
+"""
+import math
+
+import numpy as np
+
+from . import HairAsset, SimulationSettings
+
+FRAME_RATE = 60.0
+
+
+def rng_for(config, instance=0):
+    return np.random.Generator(np.random.PCG64(1000 * config + instance))
+
+
+# ---------------------------------------------------------------------------
+# Strands
+# ---------------------------------------------------------------------------
+def _perp_basis(d):
+    """Two unit vectors orthogonal to each row of d (GetTangentVectors-like)."""
+    a = np.where(np.abs(d[:, :1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t0 = np.cross(d, a)
+    t0 /= np.linalg.norm(t0, axis=1, keepdims=True)
+    t1 = np.cross(d, t0)
+    return t0, t1
+
+
+def hemisphere_roots(n, radius, rng):
+    """Area-uniform roots on the upper (+y) hemisphere of a sphere at the origin."""
+    y = rng.uniform(0.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    r = np.sqrt(np.maximum(0.0, 1.0 - y * y))
+    nrm = np.stack([r * np.cos(phi), y, r * np.sin(phi)], axis=1)
+    return radius * nrm, nrm
+
+
+def grow_strands(roots, normals, V, segment, sigma, rng):
+    """Random-walk strands: segment k's direction is segment k-1's rotated by an
+    angle ~ N(0, sigma) about a random axis perpendicular to it."""
+    G = roots.shape[0]
+    pos = np.zeros((G, V, 3))
+    pos[:, 0] = roots
+    d = normals.copy()
+    for k in range(1, V):
+        t0, t1 = _perp_basis(d)
+        ang = rng.normal(0.0, sigma, G)
+        phi = rng.uniform(0.0, 2.0 * math.pi, G)
+        axis = np.cos(phi)[:, None] * t0 + np.sin(phi)[:, None] * t1
+        # Rodrigues rotation of d about axis (axis is perpendicular to d)
+        d = d * np.cos(ang)[:, None] + np.cross(axis, d) * np.sin(ang)[:, None]
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        pos[:, k] = pos[:, k - 1] + segment * d
+    return pos
+
+
+def strand_tangents(p32):
+    """ComputeStrandTangent on float32 strands [S, V, 3] (Vector3::Normalize
+    leaves zero vectors unchanged); the last vertex keeps a zero tangent."""
+    S, V, _ = p32.shape
+
+    def nz(v):
+        d = np.sqrt((v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]) + v[..., 2] * v[..., 2])
+        out = v.copy()
+        ok = d != 0
+        out[ok] = v[ok] / d[ok][:, None]
+        return out
+
+    t = np.zeros_like(p32)
+    t[:, 0] = nz(p32[:, 1] - p32[:, 0])
+    if V > 2:
+        pre = nz(p32[:, 1:V - 1] - p32[:, 0:V - 2])
+        nxt = nz(p32[:, 2:V] - p32[:, 1:V - 1])
+        t[:, 1:V - 1] = nz((pre + nxt).astype(np.float32))
+    return t
+
+
+def build_asset(guide_pos, F, rng, follow_sigma=0.002, tip_separation=0.0, bone_index=0, skin=None):
+    """A HairAsset from guide strands [G, V, 3] (float64), with F follow strands
+    per guide (offsets ~ N(0, follow_sigma) in the root tangent plane)."""
+    G, V, _ = guide_pos.shape
+    S = G * (F + 1)
+    pos = np.zeros((G, F + 1, V, 4), np.float32)
+    gp32 = guide_pos.astype(np.float32)
+    w = np.ones(V, np.float32)
+    w[:2] = 0.0
+    pos[:, 0, :, :3] = gp32
+    pos[:, 0, :, 3] = w
+    offsets = np.zeros((G, F + 1, 4), np.float32)
+    offsets[:, :, 3] = (np.arange(G) * (F + 1)).astype(np.float32)[:, None]
+    if F > 0:
+        v01 = gp32[:, 1].astype(np.float64) - gp32[:, 0]
+        v01 /= np.linalg.norm(v01, axis=1, keepdims=True)
+        t0, t1 = _perp_basis(v01)
+        a = rng.normal(0.0, follow_sigma, (G, F))
+        b = rng.normal(0.0, follow_sigma, (G, F))
+        off = (a[..., None] * t0[:, None, :] + b[..., None] * t1[:, None, :]).astype(np.float32)
+        offsets[:, 1:, :3] = off
+        factor = (np.float32(tip_separation) * (np.arange(V, dtype=np.float32) / np.float32(V)) + np.float32(1.0))
+        pos[:, 1:, :, :3] = gp32[:, None, :, :] + off[:, :, None, :] * factor[None, None, :, None]
+        pos[:, 1:, :, 3] = w
+    pos = pos.reshape(S, V, 4)
+    # rest lengths (float32, TressFXAsset.cpp:377-399)
+    d = pos[:, :-1, :3] - pos[:, 1:, :3]
+    rl = np.zeros((S, V), np.float32)
+    rl[:, :-1] = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+    tan = np.zeros((S, V, 4), np.float32)
+    tan[..., :3] = strand_tangents(pos[..., :3])
+    if skin is None:
+        skin = np.zeros((S, 8), np.float32)
+        skin[:, 0] = bone_index
+        skin[:, 4] = 1.0
+    return HairAsset(G, F, V, pos.reshape(-1, 4), tan.reshape(-1, 4), rl.reshape(-1), offsets.reshape(-1, 4),
+                     skin)
+
+
+# ---------------------------------------------------------------------------
+# Rigid trajectories (bone matrices, row-major row-vector)
+# ---------------------------------------------------------------------------
+def yaw_pitch_matrix(yaw, pitch=0.0, translation=(0.0, 0.0, 0.0)):
+    cy, sy = math.cos(yaw), math.sin(yaw)
+    cp, sp = math.cos(pitch), math.sin(pitch)
+    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])      # column-vector R_y
+    rx = np.array([[1, 0, 0], [0, cp, -sp], [0, sp, cp]])      # column-vector R_x
+    r = ry @ rx
+    m = np.eye(4)
+    m[:3, :3] = r.T                                            # row-vector convention
+    m[3, :3] = translation
+    return m.astype(np.float32)
+
+
+def head_yaw(t, amplitude_deg=30.0, freq_hz=1.0):
+    """Config-1/2 head: rigid yaw of +-30 degrees at 1 Hz."""
+    return yaw_pitch_matrix(math.radians(amplitude_deg) * math.sin(2.0 * math.pi * freq_hz * t))
+
+
+# ---------------------------------------------------------------------------
+# Collision meshes
+# ---------------------------------------------------------------------------
+def icosphere(subdivisions):
+    """Unit icosphere, outward counter-clockwise winding. 20*4^k triangles."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, float) / np.linalg.norm(p) for p in verts]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                cache[key] = len(v) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(v)
+    Fc = np.array(faces, dtype=np.uint32)
+    # make every face wind counter-clockwise seen from outside
+    n = np.cross(V[Fc[:, 1]] - V[Fc[:, 0]], V[Fc[:, 2]] - V[Fc[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[Fc[:, 0]]) < 0
+    Fc[flip] = Fc[flip][:, [0, 2, 1]]
+    return V, Fc
+
+
+class ValueNoise3:
+    """3-D lattice value noise with smoothstep trilinear interpolation."""
+
+    def __init__(self, rng, size=64):
+        self.size = size
+        self.table = rng.uniform(-1.0, 1.0, (size, size, size))
+
+    def __call__(self, p):
+        s = self.size
+        pf = np.floor(p)
+        f = p - pf
+        i = pf.astype(np.int64) % s
+        u = f * f * (3.0 - 2.0 * f)
+        out = 0.0
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    w = ((u[:, 0] if dx else 1 - u[:, 0]) * (u[:, 1] if dy else 1 - u[:, 1]) *
+                         (u[:, 2] if dz else 1 - u[:, 2]))
+                    out = out + w * self.table[(i[:, 0] + dx) % s, (i[:, 1] + dy) % s, (i[:, 2] + dz) % s]
+        return out
+
+
+class NoisySphereMesh:
+    """Icosphere of radius r displaced along the normal by a 3-octave value
+    noise of amplitude `amp`, animated with period 1/anim_hz, then moved by a
+    rigid bone matrix.  Produces StandardVertex float32[M, 8] arrays."""
+
+    def __init__(self, rng, radius=0.1, subdivisions=5, amp=0.01, anim_hz=0.5, freq=6.0, scale=(1.0, 1.0, 1.0)):
+        self.unit, self.indices = icosphere(subdivisions)
+        self.radius = radius
+        self.amp = amp
+        self.anim_hz = anim_hz
+        self.freq = freq
+        self.scale = np.asarray(scale, float)
+        self.noise = ValueNoise3(rng)
+
+    @property
+    def num_triangles(self):
+        return self.indices.shape[0]
+
+    def vertices(self, t, bone=None):
+        ph = 2.0 * math.pi * self.anim_hz * t
+        shift = 0.5 * np.array([math.cos(ph), math.sin(ph), 0.3 * math.cos(ph)])
+        n = self.unit
+        disp = np.zeros(n.shape[0])
+        norm = 0.0
+        for o in range(3):
+            a = 0.5 ** o
+            disp += a * self.noise(n * self.freq * (2 ** o) + shift * (2 ** o) + 17.0 * o)
+            norm += a
+        r = self.radius + self.amp * disp / norm
+        p = n * self.scale * r[:, None]
+        if bone is not None:
+            ph = np.concatenate([p, np.ones((p.shape[0], 1))], axis=1) @ bone.astype(np.float64)
+            p = ph[:, :3]
+        out = np.zeros((p.shape[0], 8), np.float32)
+        out[:, :3] = p
+        out[:, 3] = 1.0
+        out[:, 4:7] = n
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Configurations
+# ---------------------------------------------------------------------------
+def settings_config1():
+    """dt 1/60, gravity 9.8, damping 0.035, global 0.1 over 0.3, local 0.8,
+    length iterations 1, VSP 0.9, wind off."""
+    s = SimulationSettings.defaults()
+    s.gravity_magnitude = 9.8
+    s.damping = 0.035
+    s.global_constraint_stiffness = 0.1
+    s.global_constraints_range = 0.3
+    s.local_constraint_stiffness = 0.8
+    s.length_constraints_iterations = 1
+    s.vsp_coeff = 0.9
+    s.wind_magnitude = 0.0
+    return s
+
+
+def settings_config2():
+    s = settings_config1()
+    s.length_constraints_iterations = 2
+    s.wind_magnitude = 3.0
+    s.wind_direction[0], s.wind_direction[1], s.wind_direction[2] = 1.0, 0.0, 0.5
+    return s
+
+
+def wind_gust(t, base=3.0):
+    """3 m/s with a +-20 % sinusoidal gust at 0.3 Hz."""
+    return base * (1.0 + 0.2 * math.sin(2.0 * math.pi * 0.3 * t))
+
+
+class Workload:
+    """One hair asset with its trajectory, collider and settings."""
+
+    def __init__(self, asset, settings, substeps, bone_fn, mesh=None, sdf_grid=None, capsules=None, wind_fn=None,
+                 collision_margin=0.5):
+        self.asset = asset
+        self.settings = settings
+        self.substeps = substeps
+        self.bone_fn = bone_fn
+        self.mesh = mesh
+        self.sdf_grid = sdf_grid          # (origin[3], cell, (nx, ny, nz))
+        self.capsules = capsules          # (c0[n,4], c1[n,4])
+        self.wind_fn = wind_fn
+        self.collision_margin = collision_margin
+
+
+def config1(num_guides=1000, V=16, seed_instance=0):
+    rng = rng_for(1, seed_instance)
+    roots, nrm = hemisphere_roots(num_guides, 0.1, rng)
+    strands = grow_strands(roots, nrm, V, 0.005, 0.15, rng)
+    asset = build_asset(strands, 0, rng)
+    eps = 1e-3
+    caps = (np.array([[0.0, 0.0, 0.0, 0.1]], np.float32), np.array([[0.0, eps, 0.0, 0.1]], np.float32))
+    return Workload(asset, settings_config1(), 1, head_yaw, capsules=caps)
+
+
+def config2(num_guides=20000, V=32, F=4, grid_n=128, cell=0.0025, subdivisions=5, seed_instance=0):
+    rng = rng_for(2, seed_instance)
+    roots, nrm = hemisphere_roots(num_guides, 0.1, rng)
+    strands = grow_strands(roots, nrm, V, 0.005, 0.15, rng)
+    asset = build_asset(strands, F, rng, follow_sigma=0.002)
+    mesh = NoisySphereMesh(rng, radius=0.1, subdivisions=subdivisions, amp=0.01, anim_hz=0.5)
+    half = 0.5 * grid_n * cell
+    grid = ((-half, -half, -half), cell, (grid_n, grid_n, grid_n))
+    return Workload(asset, settings_config2(), 2, head_yaw, mesh=mesh, sdf_grid=grid, wind_fn=wind_gust)
