@@ -199,6 +199,15 @@ typedef struct tfx_batch tfx_batch;
 /* Library                                                                   */
 /* ------------------------------------------------------------------------ */
 const char* tfx_last_error(void);
+/* Per-pass GPU timing, the counterpart of the reference's
+ * GetDevice()->GetTimeStamp(name) after each dispatch (TressFXSimulation.cpp:87-106,
+ * TressFXSDFCollision.cpp:130-209).  While enabled, HIP events bracket every
+ * kernel launch; tfx_profiling_read waits for and sums the launches recorded
+ * under `label` (a stage name, "FusedSubstep", "InitializeSignedDistanceField",
+ * "ConstructSignedDistanceField", "FinalizeSignedDistanceField",
+ * "CollideHairVerticesWithSdf") and forgets them. */
+int tfx_profiling_enable(int enable);
+int tfx_profiling_read(const char* label, double* total_ms, int* launches);
 int tfx_get_version(int* major, int* minor, int* patch); /* AMD_TressFX.h:33-35 */
 /* Selects the HIP device used by objects created afterwards on this thread. */
 int tfx_init(int hip_device);

@@ -13,8 +13,8 @@ import pytest
 import oracle
 import tressfx_amd as tfx
 from tressfx_amd import synth
-from tfx_testutil import (REL_TOL, DeviceMesh, bits_equal, first_mismatch, oracle_pair, rel_err,
-                          set_substep_inputs)
+from tfx_testutil import (REL_TOL, DeviceMesh, bits_equal, check_fp64, conditioned_tolerance, first_mismatch,
+                          oracle_pair, rel_err, set_substep_inputs)
 
 pytestmark = pytest.mark.gpu
 
@@ -90,25 +90,25 @@ def test_substeps_vs_oracles(gpu, V, F, wind):
     wl = small_workload(128, V, F, wind=wind)
     hair = make_hair(wl)
     sim = tfx.TressFXSimulation()
-    o32, o64 = oracle_pair(wl.asset)
-    worst = 0.0
+    o32 = oracle.OracleHair(wl.asset, np.float32)
+    worst, widened, total = 0.0, 0, 0
     for frame in range(8):
         for sub in range(wl.substeps):
             set_substep_inputs(hair, wl, frame, sub)
             params, bones = hair.simulation_params()
             st0 = hair.download_state()
             o32.set_state(st0)
-            o64.set_state(st0)
+            ref, tol = conditioned_tolerance(wl.asset, st0, lambda h: h.simulate(params, bones), seed=frame)
             sim.Simulate([hair])
             o32.simulate(params, bones)
-            o64.simulate(params, bones)
             st = hair.download_state()
             compare_state_fp32(st, o32, f"frame {frame} sub {sub}")
-            for k in ("pos", "prev"):
-                e = rel_err(st[k], getattr(o64, k))
-                worst = max(worst, float(e.max()))
-                assert e.max() <= REL_TOL, f"frame {frame} sub {sub}: {k} rel err {e.max():.3e} at {int(e.argmax())}"
-    print(f"worst relative error vs FP64: {worst:.3e}")
+            w, n = check_fp64(st, ref, tol, f"frame {frame} sub {sub}")
+            worst = max(worst, w)
+            widened += n
+            total += 2 * st["pos"].shape[0]
+    print(f"worst relative error vs FP64: {worst:.3e}; conditioning-widened vertex checks: {widened}/{total}")
+    assert widened <= 0.01 * total
 
 
 def test_reset_and_frame_counter(gpu):
@@ -205,7 +205,7 @@ def test_collide_and_fused_vs_oracle(gpu):
     hair_b = tfx.TressFXHairObject(wl.asset)   # fused
     desc = dm.update(0.0)
     sdf = tfx.TressFXSDFCollision(desc, collision_margin=wl.collision_margin, explicit=(origin, cell, dims))
-    o32, o64 = oracle_pair(wl.asset)
+    o32 = oracle.OracleHair(wl.asset, np.float32)
     G, F, V = wl.asset.num_guide_strands, wl.asset.num_follow_strands_per_guide, wl.asset.num_vertices_per_strand
     guide_v01 = ((np.arange(G) * (F + 1))[:, None] * V + np.arange(2)[None, :]).reshape(-1)
     hits = 0
@@ -221,28 +221,29 @@ def test_collide_and_fused_vs_oracle(gpu):
             st0 = hair_a.download_state()
             hair_b.upload_state(st0)
             o32.set_state(st0)
-            o64.set_state(st0)
+            cp = sdf.collision_params(hair_a)
+            g64 = grid32.astype(np.float64)
+
+            def advance64(h64):
+                h64.simulate(params, bones)
+                oracle.sdf_collide(h64.pos, h64.prev, V, g64, np.array(cp.origin[:3], np.float64),
+                                   float(cp.cell_size), dims, float(cp.collision_margin))
+
+            ref, tol = conditioned_tolerance(wl.asset, st0, advance64, seed=frame)
             sim.Simulate([hair_a])
             sdf.CollideWithHair(system, hair_a)
             sim.SimulateFused([hair_b], [sdf])
             o32.simulate(params, bones)
-            o64.simulate(params, bones)
-            cp = sdf.collision_params(hair_a)
             before = o32.pos.copy()
             oracle.sdf_collide(o32.pos, o32.prev, V, grid32, np.array(cp.origin[:3], np.float32), np.float32(cp.cell_size),
                                dims, np.float32(cp.collision_margin))
             hits += int((before != o32.pos).any(axis=1).sum())
-            oracle.sdf_collide(o64.pos, o64.prev, V, grid32.astype(np.float64),
-                               np.array(cp.origin[:3], np.float64), float(cp.cell_size), dims,
-                               float(cp.collision_margin))
             sa = hair_a.download_state()
             sb = hair_b.download_state()
             compare_state_fp32(sa, o32, f"plain frame {frame} sub {sub}")
             compare_state_fp32(sb, o32, f"fused frame {frame} sub {sub}", keys=("pos", "prev", "tangent"))
             assert bits_equal(sb["prevprev"][guide_v01], o32.prevprev[guide_v01])
-            for k in ("pos", "prev"):
-                e = rel_err(sa[k], getattr(o64, k))
-                assert e.max() <= REL_TOL, f"frame {frame} sub {sub}: {k} rel err {e.max():.3e}"
+            check_fp64(sa, ref, tol, f"frame {frame} sub {sub}")
     assert hits > 0, "the SDF never moved a vertex: the test does not exercise the collision"
 
 

@@ -186,6 +186,8 @@ _F = ctypes.c_float
 EXPORTS = {
     "tfx_last_error": (ctypes.c_char_p, []),
     "tfx_get_version": (_I, [ctypes.POINTER(_I)] * 3),
+    "tfx_profiling_enable": (_I, [_I]),
+    "tfx_profiling_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
     "tfx_init": (_I, [_I]),
     "tfx_simulation_settings_default": (None, [ctypes.POINTER(SimulationSettings)]),
     "tfx_hair_create": (_I, [ctypes.POINTER(AssetDesc), ctypes.POINTER(_VP)]),
@@ -256,6 +258,18 @@ def init(device=0):
     lib = load_library()
     _check(lib.tfx_init(device))
     return lib
+
+
+def profiling_enable(on=True):
+    _check(load_library().tfx_profiling_enable(1 if on else 0))
+
+
+def profiling_read(label):
+    """(total milliseconds, launches) recorded under `label` since the last read."""
+    ms = ctypes.c_double()
+    n = ctypes.c_int()
+    _check(load_library().tfx_profiling_read(label.encode(), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
 
 
 # ---------------------------------------------------------------------------

@@ -8,6 +8,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,7 +20,7 @@ using tfx::DevParams;
 using tfx::SdfGrid;
 
 extern "C" hipError_t tfx_launch_stage(int stage, const tfx::DevHair* h, hipStream_t stream);
-extern "C" hipError_t tfx_launch_sdf_build(uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
+extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
                                            const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream);
 extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, int V, const float* sdf,
                                              tfx::SdfGrid g, float margin, hipStream_t stream);
@@ -52,6 +54,82 @@ static int fail(int code, const char* fmt, ...) {
     do {                                                                \
         if (!(cond)) return fail(TFX_ERR_INVALID_ARGUMENT, "%s", msg);  \
     } while (0)
+
+// ---------------------------------------------------------------------------
+// Per-pass GPU timing (the role of GetDevice()->GetTimeStamp(name) after each
+// dispatch, TressFXSimulation.cpp:87-106, TressFXSDFCollision.cpp:130-164):
+// when enabled, a pair of HIP events brackets every kernel launch on its
+// stream and the elapsed times are summed per pass name.
+// ---------------------------------------------------------------------------
+namespace {
+struct TimerRec { hipEvent_t a, b; };
+std::mutex g_tmu;
+bool g_timing = false;
+std::map<std::string, std::vector<TimerRec>> g_timers;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pool_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct PassTimer {
+    const char* label;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    PassTimer(const char* l, hipStream_t st) : label(l), s(st) {
+        std::lock_guard<std::mutex> g(g_tmu);
+        if (!g_timing) return;
+        a = pool_event();
+        if (a) hipEventRecord(a, s);
+    }
+    ~PassTimer() {
+        if (!a) return;
+        std::lock_guard<std::mutex> g(g_tmu);
+        hipEvent_t b = pool_event();
+        if (!b) return;
+        hipEventRecord(b, s);
+        g_timers[label].push_back(TimerRec{a, b});
+    }
+};
+}  // namespace
+
+extern "C" int tfx_profiling_enable(int enable) {
+    std::lock_guard<std::mutex> g(g_tmu);
+    g_timing = enable != 0;
+    return TFX_OK;
+}
+
+extern "C" int tfx_profiling_read(const char* label, double* total_ms, int* launches) {
+    if (!label || !total_ms || !launches) return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_profiling_read: null argument");
+    std::vector<TimerRec> recs;
+    {
+        std::lock_guard<std::mutex> g(g_tmu);
+        auto it = g_timers.find(label);
+        if (it != g_timers.end()) { recs.swap(it->second); g_timers.erase(it); }
+    }
+    double sum = 0.0;
+    int rc = TFX_OK;
+    for (auto& r : recs) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) != hipSuccess || hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess)
+            rc = fail(TFX_ERR_HIP, "tfx_profiling_read: event query failed");
+        sum += ms;
+    }
+    {
+        std::lock_guard<std::mutex> g(g_tmu);
+        for (auto& r : recs) { g_event_pool.push_back(r.a); g_event_pool.push_back(r.b); }
+    }
+    *total_ms = sum;
+    *launches = (int)recs.size();
+    return rc;
+}
 
 template <class T> static hipError_t dalloc(T** p, size_t count) {
     *p = nullptr;
@@ -588,6 +666,10 @@ extern "C" int tfx_hair_upload_state(tfx_hair* h, void* stream, const float* pos
 // ---------------------------------------------------------------------------
 // TressFXSimulation
 // ---------------------------------------------------------------------------
+static const char* kStageNames[6] = {"IntegrationAndGlobalShapeConstraints", "CalculateStrandLevelData",
+                                     "VelocityShockPropagation", "LocalShapeConstraints",
+                                     "LengthConstriantsWindAndCollision", "UpdateFollowHairVertices"};
+
 extern "C" int tfx_simulation_create(tfx_simulation** out) {
     CHECK_ARG(out, "tfx_simulation_create: null argument");
     tfx_simulation* s = new tfx_simulation();
@@ -611,6 +693,7 @@ extern "C" int tfx_simulation_run_stage(tfx_simulation* sim, void* stream, tfx_h
     int rc = upload_params(h, nullptr, s);
     if (rc) return rc;
     DevHair d = dev_hair(h, nullptr);
+    PassTimer tm(kStageNames[stage], s);
     HIP_TRY(tfx_launch_stage(stage, &d, s));
     return TFX_OK;
 }
@@ -628,7 +711,10 @@ extern "C" int tfx_simulation_simulate(tfx_simulation* sim, void* stream, tfx_ha
         for (int i = 0; i < n; ++i) {
             DevHair d = dev_hair(hairs[i], nullptr);
             const int reps = (stage == 3) ? hairs[i]->params.cpu_local_shape_iterations : 1;
-            for (int r = 0; r < reps; ++r) HIP_TRY(tfx_launch_stage(stage, &d, s));
+            for (int r = 0; r < reps; ++r) {
+                PassTimer tm(kStageNames[stage], s);
+                HIP_TRY(tfx_launch_stage(stage, &d, s));
+            }
         }
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
@@ -705,6 +791,7 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         const int cnt = vclass_count[V];
         if (!cnt) continue;
         const int grid = std::min(cnt, cus * sim->grid_per_cu);
+        PassTimer tm("FusedSubstep", s);
         HIP_TRY(tfx_launch_fused(V, sim->variant[V], dtab, ditems + vclass_begin[V], cnt, grid, s));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
@@ -808,8 +895,13 @@ extern "C" int tfx_sdf_update(tfx_sdf* s, tfx_sdf_collision_system* sys, void* s
     if (!s->explicit_grid)
         for (int k = 0; k < 3; ++k) s->origin[k] = s->mesh.bbox_min[k] - s->pad_boundary;  // UpdateSDFGrid
     const int ncell = s->nx * s->ny * s->nz;
-    HIP_TRY(tfx_launch_sdf_build(s->cells, ncell, s->mesh.vertices, s->mesh.indices, s->mesh.num_triangles,
-                                 sdf_grid(s), (hipStream_t)stream));
+    static const char* kNames[3] = {"InitializeSignedDistanceField", "ConstructSignedDistanceField",
+                                    "FinalizeSignedDistanceField"};
+    for (int phase = 0; phase < 3; ++phase) {
+        PassTimer tm(kNames[phase], (hipStream_t)stream);
+        HIP_TRY(tfx_launch_sdf_phase(phase, s->cells, ncell, s->mesh.vertices, s->mesh.indices, s->mesh.num_triangles,
+                                     sdf_grid(s), (hipStream_t)stream));
+    }
     return TFX_OK;
 }
 
@@ -817,6 +909,7 @@ extern "C" int tfx_sdf_update(tfx_sdf* s, tfx_sdf_collision_system* sys, void* s
 extern "C" int tfx_sdf_collide_with_hair(tfx_sdf* s, tfx_sdf_collision_system* sys, void* stream, tfx_hair* h) {
     CHECK_ARG(s && sys && h, "tfx_sdf_collide_with_hair: null argument");
     if (!s->mesh.vertices || !s->mesh.indices) return TFX_OK;
+    PassTimer tm("CollideHairVerticesWithSdf", (hipStream_t)stream);
     HIP_TRY(tfx_launch_sdf_collide(h->pos, h->prev, h->N, h->V, reinterpret_cast<const float*>(s->cells), sdf_grid(s),
                                    s->margin * s->cell, (hipStream_t)stream));
     return TFX_OK;

@@ -294,8 +294,14 @@ __device__ __forceinline__ f3 sdf_cell_pos(const SdfGrid& g, int x, int y, int z
 
 __device__ __forceinline__ float lerp_(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
-// GetSignedDistance :472-527 (1e10 = outside the band or the grid)
-__device__ __forceinline__ float sdf_distance(const SdfGrid& g, const float* __restrict__ sdf, f3 p) {
+// GetSignedDistance :472-527 (1e10 = outside the band or the grid).
+// All eight corners are in the grid once the range test passed, so the loads
+// may be issued before the reference's "any corner is 1e10" exit is applied
+// (same result).  kAllCorners: issue all eight at once (one memory round
+// trip, for latency-bound callers); otherwise corner 0 first, which settles
+// the common far-from-the-surface case with one load.
+template <bool kAllCorners>
+__device__ __forceinline__ float sdf_distance_t(const SdfGrid& g, const float* __restrict__ sdf, f3 p) {
     const float INITIAL = 1e10f;
     int cx, cy, cz;
     sdf_coords(g, p, cx, cy, cz);
@@ -304,12 +310,16 @@ __device__ __forceinline__ float sdf_distance(const SdfGrid& g, const float* __r
     const int X = 1, Y = g.nx, Z = g.ny * g.nx;
     const int ids[8] = {base, base + X, base + Y, base + Y + X, base + Z, base + Z + X, base + Z + Y, base + Z + Y + X};
     float d[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        float v = sdf[ids[j]];
-        if (v == INITIAL) return INITIAL;
-        d[j] = v;
+    if (!kAllCorners) {
+        d[0] = sdf[ids[0]];
+        if (d[0] == INITIAL) return INITIAL;
     }
+#pragma unroll
+    for (int j = kAllCorners ? 0 : 1; j < 8; ++j) d[j] = sdf[ids[j]];
+    bool outside = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) outside |= (d[j] == INITIAL);
+    if (outside) return INITIAL;
     f3 cp = sdf_cell_pos(g, cx, cy, cz);
     f3 t = divs(sub(p, cp), g.cell);
     float a = lerp_(lerp_(d[0], d[1], t.x), lerp_(d[2], d[3], t.x), t.y);
@@ -317,16 +327,15 @@ __device__ __forceinline__ float sdf_distance(const SdfGrid& g, const float* __r
     return lerp_(a, b, t.z);
 }
 
-// CollideHairVerticesWithSdf_forward :546-595 for one vertex position.
-// Returns true (and the projected position) when the vertex is moved.
-__device__ __forceinline__ bool sdf_collide_vertex(const SdfGrid& g, const float* __restrict__ sdf, float margin,
-                                                   f3 p, f3& out) {
-    float distance = sdf_distance(g, sdf, p);
-    if (distance > margin) return false;
+// The forward-difference gradient step of CollideHairVerticesWithSdf_forward
+// :559-593 once the centre distance is known.
+template <bool kAllCorners>
+__device__ __forceinline__ bool sdf_project(const SdfGrid& g, const float* __restrict__ sdf, float margin, f3 p,
+                                            float distance, f3& out) {
     const float h = 0.1f * g.cell;
-    float fx = sdf_distance(g, sdf, f3{p.x + 1.0f * h, p.y + 0.0f * h, p.z + 0.0f * h});
-    float fy = sdf_distance(g, sdf, f3{p.x + 0.0f * h, p.y + 1.0f * h, p.z + 0.0f * h});
-    float fz = sdf_distance(g, sdf, f3{p.x + 0.0f * h, p.y + 0.0f * h, p.z + 1.0f * h});
+    float fx = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 1.0f * h, p.y + 0.0f * h, p.z + 0.0f * h});
+    float fy = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 0.0f * h, p.y + 1.0f * h, p.z + 0.0f * h});
+    float fz = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 0.0f * h, p.y + 0.0f * h, p.z + 1.0f * h});
     f3 grad = divs(f3{fx - distance, fy - distance, fz - distance}, h);
     f3 n = normalize3(grad);
     if (distance < margin) {
@@ -334,6 +343,33 @@ __device__ __forceinline__ bool sdf_collide_vertex(const SdfGrid& g, const float
         return true;
     }
     return false;
+}
+
+// CollideHairVerticesWithSdf_forward :546-595 for one vertex position.
+// Returns true (and the projected position) when the vertex is moved.
+template <bool kAllCorners = false>
+__device__ __forceinline__ bool sdf_collide_vertex(const SdfGrid& g, const float* __restrict__ sdf, float margin,
+                                                   f3 p, f3& out) {
+    float distance = sdf_distance_t<kAllCorners>(g, sdf, p);
+    if (distance > margin) return false;
+    return sdf_project<kAllCorners>(g, sdf, margin, p, distance, out);
+}
+
+// U independent vertices at once: every centre lookup is issued before any
+// result is used, then the gradient lookups of the vertices inside the margin.
+template <int U>
+__device__ __forceinline__ void sdf_collide_batch(const SdfGrid& g, const float* __restrict__ sdf, float margin,
+                                                  const f3 (&p)[U], const bool (&act)[U], f3 (&out)[U],
+                                                  bool (&hit)[U]) {
+    float d[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) d[u] = act[u] ? sdf_distance_t<true>(g, sdf, p[u]) : 1e10f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        hit[u] = false;
+        out[u] = p[u];
+        if (act[u] && !(d[u] > margin)) hit[u] = sdf_project<true>(g, sdf, margin, p[u], d[u], out[u]);
+    }
 }
 
 __device__ __forceinline__ void store_xyz(float4* p, f3 v) {

@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
     using S = FusedShape<V, VB>;
     constexpr int B = S::B, NPT = S::NPT, STRIDE = S::STRIDE;
     __shared__ float4 sPos[B * STRIDE];
+    __shared__ float4 sBind[B * STRIDE];  // P1->P4 rest pose rotated by the skinning quaternion; P6 guide tangents
     __shared__ float sMat[B][12];      // P0->P1 blended bone; P2->P3 vsp quat (0..3), translation+coeff (4..7)
     __shared__ float4 sSkinQ[B];       // P0->P4 skinning quaternion
     __shared__ float4 sV01[B][3];      // P1->P2 prev(v0), prev(v1), prevprev(v1)
@@ -91,6 +92,9 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
                 for (int q = 0; q < 9; ++q) m[q] = sMat[ls][q];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) tr[q] = sMat[ls][9 + q];
+                // bind pose for LocalShapeConstraints: MultQuaternionAndVector(boneQuat, rest)
+                const f3 bind = qrot(sSkinQ[ls], xyz(initialPos));
+                sBind[ls * STRIDE + v] = make_float4(bind.x, bind.y, bind.z, 0.0f);
                 f3 sk = bone_transform(m, tr, xyz(initialPos));
                 initialPos.x = sk.x; initialPos.y = sk.y; initialPos.z = sk.z;
                 float4 currentPos = shared;
@@ -162,18 +166,17 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
         if (P.lsc_passes > 0) {
             const float stiff = 0.5f * fmn(P.local_stiffness, 0.95f);
             for (int s = tt; s < nb; s += kFusedBlock) {
-                const size_t root = ((size_t)(g0 + s) * (F + 1)) * V;
-                const float4 bq = sSkinQ[s];
                 float4* sp = &sPos[s * STRIDE];
+                const float4* sb = &sBind[s * STRIDE];
                 for (int pass = 0; pass < P.lsc_passes; ++pass) {
                     float4 pm = sp[0];
                     float4 p = sp[1];
-                    f3 bindM = qrot(bq, xyz(h.rest[root + 0]));
-                    f3 bind = qrot(bq, xyz(h.rest[root + 1]));
+                    f3 bindM = xyz(sb[0]);
+                    f3 bind = xyz(sb[1]);
 #pragma unroll 2
                     for (int lv = 1; lv < V - 1; ++lv) {
                         float4 pp = sp[lv + 1];
-                        f3 bindP = qrot(bq, xyz(h.rest[root + lv + 1]));
+                        f3 bindP = xyz(sb[lv + 1]);
                         f3 lastVec = sub(xyz(p), xyz(pm));
                         f3 vecBindPose = sub(bindP, bind);
                         f3 lastVecBindPose = sub(bind, bindM);
@@ -276,7 +279,9 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
             __syncthreads();
         }
 
-        // ---- P6: tangent, clamp, follow strands, SDF collision, stores ------
+        // ---- P6a: guide tangent, clamp, SDF collision, stores ---------------
+        // Vertices are taken in groups of four so that all their SDF lookups
+        // are in flight together.
         const bool has_sdf = P.has_sdf != 0;
         SdfGrid grid;
         grid.ox = P.sdf_ox; grid.oy = P.sdf_oy; grid.oz = P.sdf_oz; grid.cell = P.sdf_cell;
@@ -284,62 +289,114 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
         const float margin = P.sdf_margin;
         const float C = P.clamp;
         const bool tw_uniform = P.tangent_w_uniform != 0;
+        const float* __restrict__ sdfp = h.sdf;
+        constexpr int U = NPT < 4 ? NPT : 4;
 #pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            const int lv = k * kFusedBlock + t;
-            const int ls = lv / V, v = lv % V;
-            if (ls < nb) {
-                const size_t strand = (size_t)(g0 + ls) * (F + 1);
-                const size_t i = strand * V + v;
-                const int base = ls * STRIDE;
-                const int ti = (v == V - 1) ? v - 1 : v;
-                const f3 tan = normalize3(sub(xyz(sPos[base + ti + 1]), xyz(sPos[base + ti])));
-                const float4 sp = sPos[base + v];
-                float4 pv = prevReg[k];
-                f3 delta = sub(xyz(sp), xyz(pv));
-                const float speedSqr = dot3(delta, delta);
-                if (speedSqr > C * C) {
-                    const float sc = (C * C) / speedSqr;
-                    delta = mul(delta, sc);
-                    f3 np = sub(xyz(sp), delta);
-                    pv.x = np.x; pv.y = np.y; pv.z = np.z;
-                }
-                if (hitMask & (1u << k)) pv = sp;
-                float4 pout = sp;
-                if (has_sdf && v >= 2) {
-                    f3 proj;
-                    if (sdf_collide_vertex(grid, h.sdf, margin, xyz(sp), proj)) {
-                        pout.x = proj.x; pout.y = proj.y; pout.z = proj.z;
-                        pv.x = proj.x; pv.y = proj.y; pv.z = proj.z;
+        for (int k0 = 0; k0 < NPT; k0 += U) {
+            f3 cp[U], outp[U];
+            bool act[U], hit[U];
+            float4 pvs[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u;
+                const int lv = k * kFusedBlock + t;
+                const int ls = lv / V, v = lv % V;
+                act[u] = false;
+                cp[u] = f3{0.f, 0.f, 0.f};
+                pvs[u] = prevReg[k];
+                if (ls < nb) {
+                    const float4 sp = sPos[ls * STRIDE + v];
+                    float4 pv = prevReg[k];
+                    f3 delta = sub(xyz(sp), xyz(pv));
+                    const float speedSqr = dot3(delta, delta);
+                    if (speedSqr > C * C) {
+                        const float sc = (C * C) / speedSqr;
+                        delta = mul(delta, sc);
+                        f3 np = sub(xyz(sp), delta);
+                        pv.x = np.x; pv.y = np.y; pv.z = np.z;
                     }
+                    if (hitMask & (1u << k)) pv = sp;
+                    pvs[u] = pv;
+                    cp[u] = xyz(sp);
+                    act[u] = has_sdf && v >= 2;
                 }
-                h.pos[i] = pout;
-                h.prev[i] = pv;
-                float tw;
-                if (tw_uniform) {
-                    tw = P.tangent_w;
-                    h.tangent[i] = make_float4(tan.x, tan.y, tan.z, tw);
-                } else {
-                    tw = h.tangent[i].w;
-                    store_xyz(&h.tangent[i], tan);
+            }
+            sdf_collide_batch<U>(grid, sdfp, margin, cp, act, outp, hit);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u;
+                const int lv = k * kFusedBlock + t;
+                const int ls = lv / V, v = lv % V;
+                if (ls < nb) {
+                    const size_t i = ((size_t)(g0 + ls) * (F + 1)) * V + v;
+                    const int base = ls * STRIDE;
+                    const int ti = (v == V - 1) ? v - 1 : v;
+                    const f3 tan = normalize3(sub(xyz(sPos[base + ti + 1]), xyz(sPos[base + ti])));
+                    float4 pout = sPos[base + v];
+                    float4 pv = pvs[u];
+                    if (hit[u]) {
+                        pout.x = outp[u].x; pout.y = outp[u].y; pout.z = outp[u].z;
+                        pv.x = outp[u].x; pv.y = outp[u].y; pv.z = outp[u].z;
+                    }
+                    h.pos[i] = pout;
+                    h.prev[i] = pv;
+                    float tw;
+                    if (tw_uniform) {
+                        tw = P.tangent_w;
+                        h.tangent[i] = make_float4(tan.x, tan.y, tan.z, tw);
+                    } else {
+                        tw = h.tangent[i].w;
+                        store_xyz(&h.tangent[i], tan);
+                    }
+                    // sBind is dead after the LSC sweep: it now carries the
+                    // guide tangent (with the w the follow strands copy) to P6b.
+                    sBind[base + v] = make_float4(tan.x, tan.y, tan.z, tw);
                 }
-                if (F > 0) {
-                    const float factor = P.tip_sep * ((float)v / (float)V) + 1.0f;
-                    const float4 ft = make_float4(tan.x, tan.y, tan.z, tw);
-                    for (int j = 1; j <= F; ++j) {
-                        const size_t fi = i + (size_t)V * j;
+            }
+        }
+
+        // ---- P6b: follow strands (UpdateFollowHairVertices + collision) -----
+        // Follow vertex (j, lv) of the block; consecutive lanes take
+        // consecutive vertices of one follow strand; four per lane at a time.
+        if (F > 0) {
+            __syncthreads();
+            const int per = nb * V;
+            const int nf = F * per;
+            const float tip = P.tip_sep;
+            for (int idx0 = t; idx0 < nf; idx0 += 4 * kFusedBlock) {
+                f3 fp[4], outp[4];
+                bool act[4], hit[4], ok[4];
+                size_t fi[4];
+                float4 ft[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int idx = idx0 + u * kFusedBlock;
+                    ok[u] = idx < nf;
+                    act[u] = false;
+                    fp[u] = f3{0.f, 0.f, 0.f};
+                    fi[u] = 0;
+                    ft[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (ok[u]) {
+                        const int j = idx / per + 1;
+                        const int lv = idx - (j - 1) * per;
+                        const int ls = lv / V, v = lv % V;
+                        const size_t strand = (size_t)(g0 + ls) * (F + 1);
+                        fi[u] = (strand + j) * V + v;
+                        const float4 sp = sPos[ls * STRIDE + v];
+                        ft[u] = sBind[ls * STRIDE + v];
+                        const float factor = tip * ((float)v / (float)V) + 1.0f;
                         const f3 off = xyz(h.follow_off[strand + j]);
-                        f3 fp = add(xyz(sp), mul(factor, off));
-                        if (has_sdf && v >= 2) {
-                            f3 proj;
-                            if (sdf_collide_vertex(grid, h.sdf, margin, fp, proj)) {
-                                fp = proj;
-                                store_xyz(&h.prev[fi], proj);
-                            }
-                        }
-                        store_xyz(&h.pos[fi], fp);
-                        h.tangent[fi] = ft;
+                        fp[u] = add(xyz(sp), mul(factor, off));
+                        act[u] = has_sdf && v >= 2;
                     }
+                }
+                sdf_collide_batch<4>(grid, sdfp, margin, fp, act, outp, hit);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (!ok[u]) continue;
+                    if (hit[u]) store_xyz(&h.prev[fi[u]], outp[u]);
+                    store_xyz(&h.pos[fi[u]], hit[u] ? outp[u] : fp[u]);
+                    h.tangent[fi[u]] = ft[u];
                 }
             }
         }

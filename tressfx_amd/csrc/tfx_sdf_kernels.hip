@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_collide(float4* __restrict__ 
     if (lv == 0 || lv == 1) return;
     const float4 hv = pos[i];
     f3 out;
-    if (sdf_collide_vertex(g, sdf, margin, xyz(hv), out)) {
+    if (sdf_collide_vertex<false>(g, sdf, margin, xyz(hv), out)) {
         store_xyz(&pos[i], out);
         store_xyz(&prev[i], out);
     }
@@ -152,18 +152,26 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_collide(float4* __restrict__ 
 
 }  // namespace tfx
 
-extern "C" hipError_t tfx_launch_sdf_build(uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
+// phase 0 = Initialize, 1 = Construct, 2 = Finalize
+extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
                                            const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream) {
     using namespace tfx;
     const int cblocks = std::min((ncell + kSdfBlock - 1) / kSdfBlock, 4096);
-    if (ncell > 0) hipLaunchKernelGGL(k_sdf_init, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
-    if (ntri > 0) {
-        constexpr int kTeam = 16;
-        const long long lanes = (long long)ntri * kTeam;
-        const int tblocks = (int)((lanes + kSdfBlock - 1) / kSdfBlock);
-        hipLaunchKernelGGL(k_sdf_construct<kTeam>, dim3(tblocks), dim3(kSdfBlock), 0, stream, sdf, verts, idx, ntri, g);
+    if (phase == 0) {
+        if (ncell > 0) hipLaunchKernelGGL(k_sdf_init, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
+    } else if (phase == 1) {
+        if (ntri > 0) {
+            constexpr int kTeam = 16;
+            const long long lanes = (long long)ntri * kTeam;
+            const int tblocks = (int)((lanes + kSdfBlock - 1) / kSdfBlock);
+            hipLaunchKernelGGL(k_sdf_construct<kTeam>, dim3(tblocks), dim3(kSdfBlock), 0, stream, sdf, verts, idx, ntri,
+                               g);
+        }
+    } else if (phase == 2) {
+        if (ncell > 0) hipLaunchKernelGGL(k_sdf_finalize, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
+    } else {
+        return hipErrorInvalidValue;
     }
-    if (ncell > 0) hipLaunchKernelGGL(k_sdf_finalize, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
     return hipGetLastError();
 }
 

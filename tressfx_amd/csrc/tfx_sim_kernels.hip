@@ -114,14 +114,16 @@ __global__ void __launch_bounds__(kBlock) k_lsc(DevHair h) {
     const float stiff = 0.5f * fmn(P.local_stiffness, 0.95f);
     const tfx_strand_level_data& d = h.sld[strand];
     const float4 bq = make_float4(d.skinning_quat[0], d.skinning_quat[1], d.skinning_quat[2], d.skinning_quat[3]);
+    // The window (i-1, i, i+1) slides along the strand; the values step i
+    // stores are exactly those step i+1 would read back.
+    float4 pm = h.pos[root];
+    float4 pos = h.pos[root + 1];
+    f3 bindM = qrot(bq, xyz(h.rest[root]));
+    f3 bind = qrot(bq, xyz(h.rest[root + 1]));
     for (int lv = 1; lv < V - 1; ++lv) {
         const size_t i = root + lv;
-        float4 pos = h.pos[i];
         float4 pp = h.pos[i + 1];
-        const float4 pm = h.pos[i - 1];
-        f3 bind = qrot(bq, xyz(h.rest[i]));
         f3 bindP = qrot(bq, xyz(h.rest[i + 1]));
-        f3 bindM = qrot(bq, xyz(h.rest[i - 1]));
         f3 lastVec = sub(xyz(pos), xyz(pm));
         f3 vecBindPose = sub(bindP, bind);
         f3 lastVecBindPose = sub(bind, bindM);
@@ -132,6 +134,8 @@ __global__ void __launch_bounds__(kBlock) k_lsc(DevHair h) {
         if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
         store_xyz(&h.pos[i], xyz(pos));
         store_xyz(&h.pos[i + 1], xyz(pp));
+        pm = pos; pos = pp;
+        bindM = bind; bind = bindP;
     }
 }
 

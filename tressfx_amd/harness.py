@@ -1,0 +1,97 @@
+"""Frame driver reproducing the reference's per-frame order
+(src/Simulation.cpp:50-131, TressFXSample.cpp:795-810):
+
+    skin/animate collision mesh -> SDF Update -> per substep:
+        UpdateBones + UpdateSimulationParameters -> Simulate -> CollideWithHair
+
+with a substep defined as one Simulate at dt/substeps followed by the
+collision, the SDF rebuilt once per frame and bones evaluated at the substep's
+end time.  Device memory for the collision meshes comes from torch (plumbing);
+all compute is the HIP library.
+"""
+import numpy as np
+
+from . import SdfMeshDesc, TressFXHairObject, TressFXSDFCollision, TressFXSDFCollisionSystem, TressFXSimulation
+from . import synth
+
+
+class ResidentMeshFrames:
+    """Collision-mesh vertex buffers for a window of frames, uploaded to HBM
+    before the timed region (StandardVertex float32[M, 8] per frame)."""
+
+    def __init__(self, torch, mesh, bone_fn, frames, frame_rate=synth.FRAME_RATE):
+        self.mesh = mesh
+        host = np.stack([mesh.vertices(f / frame_rate, bone_fn(f / frame_rate) if bone_fn else None) for f in frames])
+        self.frames = list(frames)
+        self.verts_d = torch.from_numpy(host).cuda()
+        self.idx_d = torch.from_numpy(mesh.indices.reshape(-1).astype(np.int32)).cuda()
+        p = host[..., :3]
+        self.bmin = p.min(axis=1)
+        self.bmax = p.max(axis=1)
+        self.nv = host.shape[1]
+
+    def desc(self, k):
+        d = SdfMeshDesc()
+        d.vertices = self.verts_d[k].data_ptr()
+        d.indices = self.idx_d.data_ptr()
+        d.num_vertices = self.nv
+        d.num_triangles = self.mesh.indices.shape[0]
+        for a in range(3):
+            d.bbox_min[a], d.bbox_max[a] = float(self.bmin[k][a]), float(self.bmax[k][a])
+            d.initial_bbox_min[a], d.initial_bbox_max[a] = float(self.bmin[0][a]), float(self.bmax[0][a])
+        return d
+
+
+class FrameDriver:
+    """One workload (hair object + collider) stepped frame by frame with either
+    the reference dispatch structure (`fused=False`) or the fused substep."""
+
+    def __init__(self, torch, wl, frames, fused=True, sim=None, system=None):
+        self.wl = wl
+        self.fused = fused
+        self.sim = sim or TressFXSimulation()
+        self.system = system or TressFXSDFCollisionSystem()
+        self.hair = TressFXHairObject(wl.asset)
+        if wl.capsules is not None:
+            self.hair.SetCollisionCapsules(*wl.capsules)
+        self.meshes = None
+        self.sdf = None
+        self.first = frames[0]
+        n = wl.substeps
+        dt = 1.0 / synth.FRAME_RATE
+        self.bones = {}
+        self.wind = {}
+        for f in frames:
+            for s in range(n):
+                t = (f + (s + 1) / n) * dt
+                self.bones[(f, s)] = np.ascontiguousarray(wl.bone_fn(t), np.float32).reshape(1, 16)
+                self.wind[(f, s)] = wl.wind_fn(t) if wl.wind_fn else None
+        self.dt_sub = np.float32(dt / n)
+        if wl.mesh is not None:
+            self.meshes = ResidentMeshFrames(torch, wl.mesh, wl.bone_fn, frames)
+            origin, cell, dims = wl.sdf_grid
+            self.sdf = TressFXSDFCollision(self.meshes.desc(0), collision_margin=wl.collision_margin,
+                                           explicit=(origin, cell, dims))
+
+    def frame(self, f, stream=None):
+        wl = self.wl
+        if self.sdf is not None:
+            self.sdf.SetMesh(self.meshes.desc(f - self.first))
+            self.sdf.Update(self.system, stream)
+        for s in range(wl.substeps):
+            self.hair.UpdateBoneMatrices(self.bones[(f, s)])
+            w = self.wind[(f, s)]
+            if w is not None:
+                wl.settings.wind_magnitude = w
+            self.hair.UpdateSimulationParameters(wl.settings, self.dt_sub)
+            if self.fused:
+                self.sim.SimulateFused([self.hair], [self.sdf] if self.sdf is not None else None, stream)
+            else:
+                self.sim.Simulate([self.hair], stream)
+                if self.sdf is not None:
+                    self.sdf.CollideWithHair(self.system, self.hair, stream)
+
+    @property
+    def vertices(self):
+        a = self.wl.asset
+        return a.num_total_vertices
