@@ -271,9 +271,10 @@ int tfx_simulation_run_stage(tfx_simulation* sim, void* stream, tfx_hair* hair, 
 int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, tfx_hair* const* hairs,
                                   tfx_sdf* const* sdfs, int num_hairs);
 /* Launch shape of the fused kernel for strands of `vertices_per_strand`
- * vertices: `variant` 0 = 1024-vertex strand blocks, 1 = 2048-vertex blocks;
- * `blocks_per_cu` sizes the persistent grid (0 keeps the current value).
- * Results do not depend on the shape. */
+ * vertices: `variant` 0, 1, 2 = strand blocks of 512, 1024, 2048 guide
+ * vertices (at most 128 strands), -1 = chosen per call (fewest rounds of
+ * resident blocks); `blocks_per_cu` caps the persistent grid per CU (0 = the
+ * kernel's occupancy).  Results do not depend on the shape. */
 int tfx_simulation_set_launch_shape(tfx_simulation* sim, int vertices_per_strand, int variant, int blocks_per_cu);
 
 /* ------------------------------------------------------------------------ */

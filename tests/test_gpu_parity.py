@@ -247,7 +247,7 @@ def test_collide_and_fused_vs_oracle(gpu):
     assert hits > 0, "the SDF never moved a vertex: the test does not exercise the collision"
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2])
 def test_fused_multi_object_mixed_lengths(gpu, variant):
     """Several objects of different strand lengths in one fused call (one
     persistent launch per length class) equal their plain results."""
@@ -258,7 +258,7 @@ def test_fused_multi_object_mixed_lengths(gpu, variant):
         wl = small_workload(G, V, F, seed=100 + i)
         objs.append((wl, make_hair(wl), make_hair(wl)))
     for V in (4, 8, 16, 32, 64):
-        sim_f.SetLaunchShape(V, variant, 2 if variant else 0)
+        sim_f.SetLaunchShape(V, variant, 2 if variant == 1 else 0)
     for frame in range(3):
         for sub in range(2):
             for wl, a, b in objs:

@@ -27,6 +27,7 @@ extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, i
 extern "C" hipError_t tfx_launch_fused(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
                                        int nitems, int grid, hipStream_t stream);
 extern "C" int tfx_fused_block_vertices(int V, int variant);
+extern "C" int tfx_fused_occupancy(int V, int variant);
 
 // ---------------------------------------------------------------------------
 // Errors
@@ -87,14 +88,14 @@ struct PassTimer {
         std::lock_guard<std::mutex> g(g_tmu);
         if (!g_timing) return;
         a = pool_event();
-        if (a) hipEventRecord(a, s);
+        if (a) (void)hipEventRecord(a, s);
     }
     ~PassTimer() {
         if (!a) return;
         std::lock_guard<std::mutex> g(g_tmu);
         hipEvent_t b = pool_event();
         if (!b) return;
-        hipEventRecord(b, s);
+        (void)hipEventRecord(b, s);
         g_timers[label].push_back(TimerRec{a, b});
     }
 };
@@ -173,8 +174,8 @@ struct Staging {
     }
     void release() {
         for (int k = 0; k < 2; ++k) {
-            if (ev[k]) { hipEventSynchronize(ev[k]); hipEventDestroy(ev[k]); ev[k] = nullptr; }
-            if (host[k]) { hipHostFree(host[k]); host[k] = nullptr; }
+            if (ev[k]) { (void)hipEventSynchronize(ev[k]); (void)hipEventDestroy(ev[k]); ev[k] = nullptr; }
+            if (host[k]) { (void)hipHostFree(host[k]); host[k] = nullptr; }
         }
         bytes = 0;
     }
@@ -213,8 +214,8 @@ struct tfx_simulation {
     Staging staging;
     void* table_d = nullptr;   // DevHair[] followed by WorkItem[]
     size_t table_bytes = 0;
-    int variant[65];           // launch-shape variant per V (autotuned)
-    int grid_per_cu = 4;
+    int variant[65];           // launch-shape variant per V (-1: chosen per call from occupancy)
+    int grid_per_cu = 0;       // persistent blocks per CU (0: the kernel's occupancy)
 };
 
 struct tfx_sdf_collision_system {
@@ -231,6 +232,9 @@ struct tfx_sdf {
     float pad_boundary = 0;
     int explicit_grid = 0;
     uint32_t* cells = nullptr;
+    uint8_t* bricks = nullptr;   // dilated 4^3-brick occupancy (conservative early-out of collision lookups)
+    uint32_t* brick_bits = nullptr;
+    int bx = 0, by = 0, bz = 0;
     int device = 0;
 };
 
@@ -241,9 +245,9 @@ static void set_identity_bones(std::vector<float>& b) {
 }
 
 static void hair_free(tfx_hair* h) {
-    hipFree(h->pos); hipFree(h->prev); hipFree(h->prevprev); hipFree(h->tangent);
-    hipFree(h->sld); hipFree(h->rest); hipFree(h->restlen); hipFree(h->follow_off);
-    hipFree(h->skin); hipFree(h->bones_d); hipFree(h->params_d);
+    (void)hipFree(h->pos); (void)hipFree(h->prev); (void)hipFree(h->prevprev); (void)hipFree(h->tangent);
+    (void)hipFree(h->sld); (void)hipFree(h->rest); (void)hipFree(h->restlen); (void)hipFree(h->follow_off);
+    (void)hipFree(h->skin); (void)hipFree(h->bones_d); (void)hipFree(h->params_d);
 }
 
 static void compute_tangent_w(tfx_hair* h, const float* tangents) {
@@ -302,6 +306,8 @@ static void pack_dev_params(const tfx_hair* h, const tfx_sdf* sdf, DevParams& d)
         d.has_sdf = 1;
         // m_CollisionMargin * m_CellSize (TressFXSDFCollision.cpp:194)
         d.sdf_margin = sdf->margin * sdf->cell;
+        d.sdf_inv_cell = 1.0f / sdf->cell;
+        d.sdf_bx = sdf->bx; d.sdf_by = sdf->by; d.sdf_bz = sdf->bz;
     }
     d.tangent_w_uniform = h->tangent_w_uniform;
     d.tangent_w = h->tangent_w;
@@ -334,6 +340,8 @@ static DevHair dev_hair(const tfx_hair* h, const tfx_sdf* sdf) {
     d.rest = h->rest; d.restlen = h->restlen; d.follow_off = h->follow_off; d.skin = h->skin;
     d.bones = h->bones_d; d.params = h->params_d;
     d.sdf = sdf ? reinterpret_cast<const float*>(sdf->cells) : nullptr;
+    d.sdf_bricks = sdf ? sdf->bricks : nullptr;
+    d.sdf_brick_bits = sdf ? sdf->brick_bits : nullptr;
     d.G = h->G; d.F = h->F; d.V = h->V; d.pad = 0;
     return d;
 }
@@ -343,6 +351,10 @@ static SdfGrid sdf_grid(const tfx_sdf* s) {
     g.ox = s->origin[0]; g.oy = s->origin[1]; g.oz = s->origin[2];
     g.cell = s->cell;
     g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.pad = 0;
+    g.bricks = s->bricks;
+    g.brick_bits = s->brick_bits;
+    g.inv_cell = 1.0f / s->cell;
+    g.bx = s->bx; g.by = s->by; g.bz = s->bz;
     return g;
 }
 
@@ -531,7 +543,7 @@ extern "C" int tfx_hair_create(const tfx_asset_desc* a, tfx_hair** out) {
 
 extern "C" int tfx_hair_destroy(tfx_hair* h) {
     if (!h) return TFX_OK;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     hair_free(h);
     delete h;
     return TFX_OK;
@@ -674,15 +686,15 @@ extern "C" int tfx_simulation_create(tfx_simulation** out) {
     CHECK_ARG(out, "tfx_simulation_create: null argument");
     tfx_simulation* s = new tfx_simulation();
     s->device = g_device;
-    for (int v = 0; v <= 64; ++v) s->variant[v] = 0;
+    for (int v = 0; v <= 64; ++v) s->variant[v] = -1;
     *out = s;
     return TFX_OK;
 }
 
 extern "C" int tfx_simulation_destroy(tfx_simulation* s) {
     if (!s) return TFX_OK;
-    hipDeviceSynchronize();
-    hipFree(s->table_d);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(s->table_d);
     delete s;
     return TFX_OK;
 }
@@ -723,11 +735,42 @@ extern "C" int tfx_simulation_simulate(tfx_simulation* sim, void* stream, tfx_ha
 
 extern "C" int tfx_simulation_set_launch_shape(tfx_simulation* sim, int V, int variant, int blocks_per_cu) {
     CHECK_ARG(sim && tfx_fused_block_vertices(V, 0) > 0, "tfx_simulation_set_launch_shape: bad vertices per strand");
-    CHECK_ARG(variant == 0 || variant == 1, "tfx_simulation_set_launch_shape: variant must be 0 or 1");
+    CHECK_ARG(variant >= -1 && variant <= 2, "tfx_simulation_set_launch_shape: variant must be -1 (automatic), 0, 1 or 2");
     CHECK_ARG(blocks_per_cu >= 0 && blocks_per_cu <= 64, "tfx_simulation_set_launch_shape: blocks_per_cu out of range");
     sim->variant[V] = variant;
-    if (blocks_per_cu > 0) sim->grid_per_cu = blocks_per_cu;
+    sim->grid_per_cu = blocks_per_cu;
     return TFX_OK;
+}
+
+// Launch shape of one strand-length class: the variant fixed by
+// tfx_simulation_set_launch_shape, or else the one whose resident blocks cover
+// the class's strand blocks in the fewest rounds (ties: more resident blocks).
+struct Shape { int variant; int resident; };
+
+static Shape choose_shape(const tfx_simulation* sim, int V, tfx_hair* const* hairs, int n, int cus) {
+    auto resident_of = [&](int variant) {
+        const int occ = tfx_fused_occupancy(V, variant);
+        const int per_cu = sim->grid_per_cu > 0 ? std::min(sim->grid_per_cu, std::max(occ, 1)) : std::max(occ, 1);
+        return per_cu * cus;
+    };
+    if (sim->variant[V] >= 0) return Shape{sim->variant[V], resident_of(sim->variant[V])};
+    Shape best{0, resident_of(0)};
+    long long best_rounds = -1;
+    for (int variant = 0; variant <= 2; ++variant) {
+        const int vb = tfx_fused_block_vertices(V, variant);
+        if (variant > 0 && vb == tfx_fused_block_vertices(V, variant - 1)) continue;
+        const int B = vb / V;
+        long long items = 0;
+        for (int i = 0; i < n; ++i)
+            if (hairs[i]->V == V) items += (hairs[i]->G + B - 1) / B;
+        const int res = resident_of(variant);
+        const long long rounds = (items + res - 1) / res;
+        if (best_rounds < 0 || rounds < best_rounds || (rounds == best_rounds && res > best.resident)) {
+            best = Shape{variant, res};
+            best_rounds = rounds;
+        }
+    }
+    return best;
 }
 
 static int ensure_table(tfx_simulation* sim, size_t bytes) {
@@ -761,10 +804,18 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
     }
     // Table layout: DevHair[n] | WorkItem[total]
     std::vector<tfx::WorkItem> items;
-    std::vector<int> vclass_begin(65, 0), vclass_count(65, 0);
+    std::vector<int> vclass_begin(65, 0), vclass_count(65, 0), vclass_variant(65, 0), vclass_resident(65, 0);
+    const int cus = cu_count(sim->device);
     for (int V = 4; V <= 64; V *= 2) {
         vclass_begin[V] = (int)items.size();
-        const int B = tfx_fused_block_vertices(V, sim->variant[V]) / V;
+        long long guides = 0;
+        for (int i = 0; i < n; ++i)
+            if (hairs[i]->V == V) guides += hairs[i]->G;
+        if (!guides) continue;
+        const Shape sh = choose_shape(sim, V, hairs, n, cus);
+        vclass_variant[V] = sh.variant;
+        vclass_resident[V] = sh.resident;
+        const int B = tfx_fused_block_vertices(V, sh.variant) / V;
         for (int i = 0; i < n; ++i)
             if (hairs[i]->V == V)
                 for (int g0 = 0; g0 < hairs[i]->G; g0 += B) items.push_back(tfx::WorkItem{i, g0});
@@ -786,13 +837,12 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
     if ((rc = sim->staging.release_slot(slot, s))) return rc;
     const DevHair* dtab = (const DevHair*)sim->table_d;
     const tfx::WorkItem* ditems = (const tfx::WorkItem*)((char*)sim->table_d + item_off);
-    const int cus = cu_count(sim->device);
     for (int V = 4; V <= 64; V *= 2) {
         const int cnt = vclass_count[V];
         if (!cnt) continue;
-        const int grid = std::min(cnt, cus * sim->grid_per_cu);
+        const int grid = std::min(cnt, vclass_resident[V]);
         PassTimer tm("FusedSubstep", s);
-        HIP_TRY(tfx_launch_fused(V, sim->variant[V], dtab, ditems + vclass_begin[V], cnt, grid, s));
+        HIP_TRY(tfx_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, s));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
     return TFX_OK;
@@ -821,6 +871,19 @@ static int sdf_alloc(tfx_sdf* s) {
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_sdf: %s", hipGetErrorString(e));
     HIP_TRY(hipMemset(s->cells, 0, (size_t)s->total_cells * sizeof(uint32_t)));
+    s->bx = (s->nx + (1 << tfx::kBrickShift) - 1) >> tfx::kBrickShift;
+    s->by = (s->ny + (1 << tfx::kBrickShift) - 1) >> tfx::kBrickShift;
+    s->bz = (s->nz + (1 << tfx::kBrickShift) - 1) >> tfx::kBrickShift;
+    e = hipMalloc((void**)&s->bricks, (size_t)s->bx * s->by * s->bz);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_sdf: %s", hipGetErrorString(e));
+    const size_t nbrick = (size_t)s->bx * s->by * s->bz;
+    e = hipMalloc((void**)&s->brick_bits, ((nbrick + 31) / 32) * sizeof(uint32_t));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_sdf: %s", hipGetErrorString(e));
+    // Until the first Update every brick is "maybe occupied": lookups stay exact.
+    HIP_TRY(hipMemset(s->bricks, 1, nbrick));
+    HIP_TRY(hipMemset(s->brick_bits, 0xff, ((nbrick + 31) / 32) * sizeof(uint32_t)));
     return TFX_OK;
 }
 
@@ -876,8 +939,10 @@ extern "C" int tfx_sdf_create_explicit(const tfx_sdf_mesh_desc* mesh, const floa
 
 extern "C" int tfx_sdf_destroy(tfx_sdf* s) {
     if (!s) return TFX_OK;
-    hipDeviceSynchronize();
-    hipFree(s->cells);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(s->cells);
+    (void)hipFree(s->bricks);
+    (void)hipFree(s->brick_bits);
     delete s;
     return TFX_OK;
 }
@@ -895,9 +960,9 @@ extern "C" int tfx_sdf_update(tfx_sdf* s, tfx_sdf_collision_system* sys, void* s
     if (!s->explicit_grid)
         for (int k = 0; k < 3; ++k) s->origin[k] = s->mesh.bbox_min[k] - s->pad_boundary;  // UpdateSDFGrid
     const int ncell = s->nx * s->ny * s->nz;
-    static const char* kNames[3] = {"InitializeSignedDistanceField", "ConstructSignedDistanceField",
-                                    "FinalizeSignedDistanceField"};
-    for (int phase = 0; phase < 3; ++phase) {
+    static const char* kNames[4] = {"InitializeSignedDistanceField", "ConstructSignedDistanceField",
+                                    "FinalizeSignedDistanceField", "FinalizeSignedDistanceField"};
+    for (int phase = 0; phase < 4; ++phase) {
         PassTimer tm(kNames[phase], (hipStream_t)stream);
         HIP_TRY(tfx_launch_sdf_phase(phase, s->cells, ncell, s->mesh.vertices, s->mesh.indices, s->mesh.num_triangles,
                                      sdf_grid(s), (hipStream_t)stream));

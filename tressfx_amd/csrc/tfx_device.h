@@ -243,7 +243,8 @@ struct alignas(16) DevParams {
     float sdf_margin;
     int tangent_w_uniform;  // all guide tangent w equal -> tangent_w
     float tangent_w;
-    int pad;
+    float sdf_inv_cell;
+    int sdf_bx, sdf_by, sdf_bz, pad2;
 };
 
 // Device pointers and counts of one hair object.
@@ -260,6 +261,8 @@ struct DevHair {
     const float* bones;          // 512 x 16
     const DevParams* params;
     const float* sdf;            // float cells of the hair's SDF (fused path)
+    const uint8_t* sdf_bricks;   // its dilated brick occupancy
+    const uint32_t* sdf_brick_bits;  // the same, one bit per brick
     int G, F, V, pad;
 };
 
@@ -274,7 +277,17 @@ struct WorkItem {
 struct SdfGrid {
     float ox, oy, oz, cell;
     int nx, ny, nz, pad;
+    // Occupancy of 4^3-cell bricks, dilated by kBrickHalo cells: brick b is
+    // non-zero if any cell within kBrickHalo cells of it holds a distance
+    // other than 1e10.  Null = no map (every lookup is exact).
+    const uint8_t* bricks;
+    const uint32_t* brick_bits; // the same occupancy, one bit per brick
+    float inv_cell;             // 1 / cell, for the conservative brick test only
+    int bx, by, bz;             // bricks per axis
 };
+
+constexpr int kBrickShift = 2;  // 4^3 cells per brick
+constexpr int kBrickHalo = 2;
 
 // GetSdfCoordinates :103-113
 __device__ __forceinline__ void sdf_coords(const SdfGrid& g, f3 p, int& cx, int& cy, int& cz) {
@@ -300,6 +313,30 @@ __device__ __forceinline__ float lerp_(float a, float b, float t) { return a * (
 // (same result).  kAllCorners: issue all eight at once (one memory round
 // trip, for latency-bound callers); otherwise corner 0 first, which settles
 // the common far-from-the-surface case with one load.
+// Conservative test: false only if the exact lookup at p reads no cell other
+// than 1e10 (or leaves the grid).  The approximate cell index (p - o) * 1/cell
+// is within one cell of the exact ftoi((p - o) / cell); the lookup reads cells
+// cx..cx+1 per axis, all within the brick halo of the approximate cell's brick.
+__device__ __forceinline__ bool sdf_brick_of(const SdfGrid& g, f3 p, int& b) {
+    const float ax = floorf((p.x - g.ox) * g.inv_cell);
+    const float ay = floorf((p.y - g.oy) * g.inv_cell);
+    const float az = floorf((p.z - g.oz) * g.inv_cell);
+    // Outside [-2, n] the exact index is outside [0, n-3]: GetSignedDistance returns 1e10.
+    if (!(ax >= -2.0f && ax <= (float)g.nx && ay >= -2.0f && ay <= (float)g.ny && az >= -2.0f && az <= (float)g.nz))
+        return false;
+    const int bxi = min(max((int)ax, 0) >> kBrickShift, g.bx - 1);
+    const int byi = min(max((int)ay, 0) >> kBrickShift, g.by - 1);
+    const int bzi = min(max((int)az, 0) >> kBrickShift, g.bz - 1);
+    b = (bzi * g.by + byi) * g.bx + bxi;
+    return true;
+}
+
+__device__ __forceinline__ bool sdf_maybe_near(const SdfGrid& g, f3 p) {
+    if (!g.bricks) return true;
+    int b;
+    return sdf_brick_of(g, p, b) && g.bricks[b] != 0;
+}
+
 template <bool kAllCorners>
 __device__ __forceinline__ float sdf_distance_t(const SdfGrid& g, const float* __restrict__ sdf, f3 p) {
     const float INITIAL = 1e10f;
@@ -350,6 +387,7 @@ __device__ __forceinline__ bool sdf_project(const SdfGrid& g, const float* __res
 template <bool kAllCorners = false>
 __device__ __forceinline__ bool sdf_collide_vertex(const SdfGrid& g, const float* __restrict__ sdf, float margin,
                                                    f3 p, f3& out) {
+    if (!sdf_maybe_near(g, p)) return false;  // the exact distance is 1e10
     float distance = sdf_distance_t<kAllCorners>(g, sdf, p);
     if (distance > margin) return false;
     return sdf_project<kAllCorners>(g, sdf, margin, p, distance, out);
@@ -363,7 +401,7 @@ __device__ __forceinline__ void sdf_collide_batch(const SdfGrid& g, const float*
                                                   bool (&hit)[U]) {
     float d[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) d[u] = act[u] ? sdf_distance_t<true>(g, sdf, p[u]) : 1e10f;
+    for (int u = 0; u < U; ++u) d[u] = (act[u] && sdf_maybe_near(g, p[u])) ? sdf_distance_t<true>(g, sdf, p[u]) : 1e10f;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         hit[u] = false;

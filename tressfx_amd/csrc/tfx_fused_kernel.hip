@@ -24,6 +24,8 @@
 namespace tfx {
 
 constexpr int kFusedBlock = 256;
+constexpr int kMaxStagedFollow = 4;  // follow strands per guide whose offsets are staged in LDS
+constexpr int kMaxBrickWords = 1024; // SDF grids of up to 32768 bricks keep their occupancy in LDS
 
 template <int V, int VB>
 struct FusedShape {
@@ -35,15 +37,23 @@ struct FusedShape {
 };
 
 template <int V, int VB>
-__global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __restrict__ hairs,
+__global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair* __restrict__ hairs,
                                                                const WorkItem* __restrict__ items, int nitems) {
     using S = FusedShape<V, VB>;
     constexpr int B = S::B, NPT = S::NPT, STRIDE = S::STRIDE;
     __shared__ float4 sPos[B * STRIDE];
-    __shared__ float4 sBind[B * STRIDE];  // P1->P4 rest pose rotated by the skinning quaternion; P6 guide tangents
+    // P1->P4: bind-pose vectors of LocalShapeConstraints (:799-807) per vertex i,
+    //   vecBindPose = bind[i+1] - bind[i] in sVB[i].xyz, n = normalize(bind[i] - bind[i-1])
+    //   in (sVB[i].w, sNB[i].x, sNB[i].y), bind = MultQuaternionAndVector(skinningQuat, rest).
+    //   P6: sVB carries the guide tangents.
+    __shared__ float4 sVB[B * STRIDE];
+    __shared__ float2 sNB[B * STRIDE];
     __shared__ float sMat[B][12];      // P0->P1 blended bone; P2->P3 vsp quat (0..3), translation+coeff (4..7)
-    __shared__ float4 sSkinQ[B];       // P0->P4 skinning quaternion
+    __shared__ float4 sSkinQ[B];       // P0->P1 skinning quaternion
     __shared__ float4 sV01[B][3];      // P1->P2 prev(v0), prev(v1), prevprev(v1)
+    __shared__ float4 sOff[B][kMaxStagedFollow];  // P0->P6b follow root offsets (F <= kMaxStagedFollow)
+    __shared__ uint32_t sBits[kMaxBrickWords];    // SDF brick occupancy, one bit per brick
+    __shared__ int sQueueCount;
 
     const int t = threadIdx.x;
     // Lane-per-strand phases use threads tt = t rotated by a block-dependent
@@ -59,6 +69,13 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
         const int F = h.F;
         const int nb = min(B, h.G - g0);
 
+        // SDF brick occupancy into LDS (read by P6's conservative early-out).
+        const bool has_sdf = P.has_sdf != 0;
+        const int brick_words = (P.sdf_bx * P.sdf_by * P.sdf_bz + 31) >> 5;
+        const bool bits_lds = has_sdf && h.sdf_brick_bits != nullptr && brick_words <= kMaxBrickWords;
+        if (bits_lds)
+            for (int k = t; k < brick_words; k += kFusedBlock) sBits[k] = h.sdf_brick_bits[k];
+
         // ---- P0: blended bone per strand ---------------------------------
         for (int s = tt; s < nb; s += kFusedBlock) {
             const size_t strand = (size_t)(g0 + s) * (F + 1);
@@ -69,6 +86,8 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
 #pragma unroll
             for (int k = 0; k < 3; ++k) sMat[s][9 + k] = bb.t[k];
             sSkinQ[s] = quat_from_matrix3(bb.m);
+            if (F <= kMaxStagedFollow)
+                for (int j = 0; j < F; ++j) sOff[s][j] = h.follow_off[strand + 1 + j];
         }
         __syncthreads();
 
@@ -82,9 +101,21 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
             const int lv = k * kFusedBlock + t;
             const int ls = lv / V, v = lv % V;
             prevReg[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ls < nb) {
-                const size_t i = ((size_t)(g0 + ls) * (F + 1)) * V + v;
-                float4 initialPos = h.rest[i];
+            const bool valid = ls < nb;
+            const size_t i = ((size_t)(g0 + (valid ? ls : 0)) * (F + 1)) * V + v;
+            float4 initialPos = valid ? h.rest[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            // Bind pose of LocalShapeConstraints, MultQuaternionAndVector(boneQuat,
+            // rest) (:799-801); a strand's vertices sit in consecutive lanes of one
+            // wave (64 % V == 0), so the neighbours come from lane +-1.
+            const f3 bind = qrot(sSkinQ[valid ? ls : 0], xyz(initialPos));
+            const f3 bindP{__shfl_down(bind.x, 1), __shfl_down(bind.y, 1), __shfl_down(bind.z, 1)};
+            const f3 bindM{__shfl_up(bind.x, 1), __shfl_up(bind.y, 1), __shfl_up(bind.z, 1)};
+            if (valid) {
+                const f3 vb = sub(bindP, bind);
+                f3 nbp{0.f, 0.f, 0.f};
+                if (v >= 1 && v < V - 1) nbp = normalize3(sub(bind, bindM));
+                sVB[ls * STRIDE + v] = make_float4(vb.x, vb.y, vb.z, nbp.x);
+                sNB[ls * STRIDE + v] = make_float2(nbp.y, nbp.z);
                 float4 shared = h.pos[i];
                 const float4 prevOld = h.prev[i];
                 float m[9], tr[3];
@@ -92,9 +123,6 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
                 for (int q = 0; q < 9; ++q) m[q] = sMat[ls][q];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) tr[q] = sMat[ls][9 + q];
-                // bind pose for LocalShapeConstraints: MultQuaternionAndVector(boneQuat, rest)
-                const f3 bind = qrot(sSkinQ[ls], xyz(initialPos));
-                sBind[ls * STRIDE + v] = make_float4(bind.x, bind.y, bind.z, 0.0f);
                 f3 sk = bone_transform(m, tr, xyz(initialPos));
                 initialPos.x = sk.x; initialPos.y = sk.y; initialPos.z = sk.z;
                 float4 currentPos = shared;
@@ -163,34 +191,56 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
         __syncthreads();
 
         // ---- P4: local shape constraints (Gauss–Seidel sweeps) -------------
+        // Pass k of the sweep at vertex i reads vertices i-1..i+1 and writes
+        // i, i+1; once pass k has completed step i+2 nothing it does later
+        // touches vertices <= i+1.  So up to kLscPipe passes run together, pass
+        // k on its own lane lagging kLscLag steps behind pass k-1: every step
+        // reads exactly the values the sequential order gives it.  The lanes
+        // of one strand are adjacent in one wave; the wave runs the steps in
+        // lock step and orders its LDS accesses by a wave-scope fence per step.
         if (P.lsc_passes > 0) {
             const float stiff = 0.5f * fmn(P.local_stiffness, 0.95f);
-            for (int s = tt; s < nb; s += kFusedBlock) {
+            constexpr int kLscPipe = 4, kLscLag = 3;
+            const int npass = P.lsc_passes;
+            const int lanes = npass < kLscPipe ? npass : kLscPipe;
+            const int steps = (V - 2) + kLscLag * (lanes - 1);
+            for (int item = tt; item < nb * lanes; item += kFusedBlock) {
+                const int s = item / lanes, j = item - s * lanes;
                 float4* sp = &sPos[s * STRIDE];
-                const float4* sb = &sBind[s * STRIDE];
-                for (int pass = 0; pass < P.lsc_passes; ++pass) {
-                    float4 pm = sp[0];
-                    float4 p = sp[1];
-                    f3 bindM = xyz(sb[0]);
-                    f3 bind = xyz(sb[1]);
-#pragma unroll 2
-                    for (int lv = 1; lv < V - 1; ++lv) {
-                        float4 pp = sp[lv + 1];
-                        f3 bindP = xyz(sb[lv + 1]);
-                        f3 lastVec = sub(xyz(p), xyz(pm));
-                        f3 vecBindPose = sub(bindP, bind);
-                        f3 lastVecBindPose = sub(bind, bindM);
-                        float4 rg = quat_two_vectors(normalize3(lastVecBindPose), normalize3(lastVec));
-                        f3 org = add(qrot(rg, vecBindPose), xyz(p));
-                        f3 del = mul(stiff, sub(org, xyz(pp)));
-                        if (movable(p)) { p.x -= del.x; p.y -= del.y; p.z -= del.z; }
-                        if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
-                        sp[lv] = p;
-                        pm = p; p = pp;
-                        bindM = bind; bind = bindP;
+                const float4* svb = &sVB[s * STRIDE];
+                const float2* snb = &sNB[s * STRIDE];
+                // The sweep is one dependent chain per strand: raise this
+                // wave's issue priority while it runs.
+                __builtin_amdgcn_s_setprio(3);
+                for (int group = 0; group < npass; group += lanes) {
+                    const bool live = group + j < npass;
+                    const int lag = kLscLag * j;
+                    float4 pm = make_float4(0.f, 0.f, 0.f, 0.f), p = pm;
+                    for (int it = 1; it <= steps; ++it) {
+                        const int lv = it - lag;
+                        if (live && lv >= 1 && lv <= V - 2) {
+                            if (lv == 1) { pm = sp[0]; p = sp[1]; }
+                            float4 pp = sp[lv + 1];
+                            const float4 vbw = svb[lv];
+                            const float2 nyz = snb[lv];
+                            const f3 vecBindPose = xyz(vbw);
+                            const f3 nLastBind{vbw.w, nyz.x, nyz.y};
+                            f3 lastVec = sub(xyz(p), xyz(pm));
+                            float4 rg = quat_two_vectors(nLastBind, normalize3(lastVec));
+                            f3 org = add(qrot(rg, vecBindPose), xyz(p));
+                            f3 del = mul(stiff, sub(org, xyz(pp)));
+                            if (movable(p)) { p.x -= del.x; p.y -= del.y; p.z -= del.z; }
+                            if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
+                            sp[lv] = p;
+                            if (lv == V - 2) sp[V - 1] = pp;
+                            pm = p; p = pp;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     }
-                    sp[V - 1] = p;
                 }
+                __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
         }
@@ -281,11 +331,20 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
 
         // ---- P6a: guide tangent, clamp, SDF collision, stores ---------------
         // Vertices are taken in groups of four so that all their SDF lookups
-        // are in flight together.
-        const bool has_sdf = P.has_sdf != 0;
+        // are in flight together.  A vertex whose brick neighbourhood holds no
+        // distance other than 1e10 skips the lookup (it would return 1e10).
         SdfGrid grid;
         grid.ox = P.sdf_ox; grid.oy = P.sdf_oy; grid.oz = P.sdf_oz; grid.cell = P.sdf_cell;
         grid.nx = P.sdf_nx; grid.ny = P.sdf_ny; grid.nz = P.sdf_nz; grid.pad = 0;
+        grid.bricks = bits_lds ? nullptr : h.sdf_bricks;
+        grid.brick_bits = nullptr;
+        grid.inv_cell = P.sdf_inv_cell;
+        grid.bx = P.sdf_bx; grid.by = P.sdf_by; grid.bz = P.sdf_bz;
+        auto maybe_near = [&](f3 p) -> bool {
+            if (!bits_lds) return true;
+            int b;
+            return sdf_brick_of(grid, p, b) && ((sBits[b >> 5] >> (b & 31)) & 1u);
+        };
         const float margin = P.sdf_margin;
         const float C = P.clamp;
         const bool tw_uniform = P.tangent_w_uniform != 0;
@@ -318,7 +377,7 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
                     if (hitMask & (1u << k)) pv = sp;
                     pvs[u] = pv;
                     cp[u] = xyz(sp);
-                    act[u] = has_sdf && v >= 2;
+                    act[u] = has_sdf && v >= 2 && maybe_near(cp[u]);
                 }
             }
             sdf_collide_batch<U>(grid, sdfp, margin, cp, act, outp, hit);
@@ -348,46 +407,74 @@ __global__ void __launch_bounds__(kFusedBlock) k_fused_substep(const DevHair* __
                         tw = h.tangent[i].w;
                         store_xyz(&h.tangent[i], tan);
                     }
-                    // sBind is dead after the LSC sweep: it now carries the
+                    // sVB is dead after the LSC sweep: it now carries the
                     // guide tangent (with the w the follow strands copy) to P6b.
-                    sBind[base + v] = make_float4(tan.x, tan.y, tan.z, tw);
+                    sVB[base + v] = make_float4(tan.x, tan.y, tan.z, tw);
                 }
             }
         }
 
         // ---- P6b: follow strands (UpdateFollowHairVertices + collision) -----
-        // Follow vertex (j, lv) of the block; consecutive lanes take
-        // consecutive vertices of one follow strand; four per lane at a time.
+        // Follow vertex idx = (j - 1) * nb * V + lv of the block; consecutive
+        // lanes take consecutive vertices of one follow strand.  Pass 1 stores
+        // every vertex that provably stays put and queues the others (in the
+        // dead sNB storage); pass 2 runs the SDF projection densely over the
+        // queue, four vertices per lane at a time.
         if (F > 0) {
+            if (t == 0) sQueueCount = 0;
             __syncthreads();
             const int per = nb * V;
             const int nf = F * per;
             const float tip = P.tip_sep;
-            for (int idx0 = t; idx0 < nf; idx0 += 4 * kFusedBlock) {
+            const bool staged = F <= kMaxStagedFollow;
+            uint16_t* queue = reinterpret_cast<uint16_t*>(sNB);   // 4 * B * STRIDE >= F * VB entries when staged
+            auto follow_vertex = [&](int idx, size_t& fi, f3& fp, float4& ft, int& v) {
+                const int j = idx / per + 1;
+                const int lv = idx - (j - 1) * per;
+                const int ls = lv / V;
+                v = lv % V;
+                const size_t strand = (size_t)(g0 + ls) * (F + 1);
+                fi = (strand + j) * V + v;
+                const float4 sp = sPos[ls * STRIDE + v];
+                ft = sVB[ls * STRIDE + v];
+                const float factor = tip * ((float)v / (float)V) + 1.0f;
+                const f3 off = xyz(staged ? sOff[ls][j - 1] : h.follow_off[strand + j]);
+                fp = add(xyz(sp), mul(factor, off));
+            };
+            if (staged) {
+                for (int idx = t; idx < nf; idx += kFusedBlock) {
+                    size_t fi;
+                    f3 fp;
+                    float4 ft;
+                    int v;
+                    follow_vertex(idx, fi, fp, ft, v);
+                    if (has_sdf && v >= 2 && maybe_near(fp)) {
+                        queue[atomicAdd(&sQueueCount, 1)] = (uint16_t)idx;
+                    } else {
+                        store_xyz(&h.pos[fi], fp);
+                        h.tangent[fi] = ft;
+                    }
+                }
+                __syncthreads();
+            }
+            const int nq = staged ? sQueueCount : nf;
+            for (int q0 = t; q0 < nq; q0 += 4 * kFusedBlock) {
                 f3 fp[4], outp[4];
                 bool act[4], hit[4], ok[4];
                 size_t fi[4];
                 float4 ft[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int idx = idx0 + u * kFusedBlock;
-                    ok[u] = idx < nf;
+                    const int q = q0 + u * kFusedBlock;
+                    ok[u] = q < nq;
                     act[u] = false;
                     fp[u] = f3{0.f, 0.f, 0.f};
                     fi[u] = 0;
                     ft[u] = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (ok[u]) {
-                        const int j = idx / per + 1;
-                        const int lv = idx - (j - 1) * per;
-                        const int ls = lv / V, v = lv % V;
-                        const size_t strand = (size_t)(g0 + ls) * (F + 1);
-                        fi[u] = (strand + j) * V + v;
-                        const float4 sp = sPos[ls * STRIDE + v];
-                        ft[u] = sBind[ls * STRIDE + v];
-                        const float factor = tip * ((float)v / (float)V) + 1.0f;
-                        const f3 off = xyz(h.follow_off[strand + j]);
-                        fp[u] = add(xyz(sp), mul(factor, off));
-                        act[u] = has_sdf && v >= 2;
+                        int v;
+                        follow_vertex(staged ? (int)queue[q] : q, fi[u], fp[u], ft[u], v);
+                        act[u] = has_sdf && v >= 2 && (staged || maybe_near(fp[u]));
                     }
                 }
                 sdf_collide_batch<4>(grid, sdfp, margin, fp, act, outp, hit);
@@ -413,33 +500,46 @@ static hipError_t launch_fused(const DevHair* hairs, const WorkItem* items, int 
 
 }  // namespace tfx
 
-// Vertices per block for each strand length and launch-shape variant
-// (variant 0: 1024 vertices, no register spills; variant 1: 2048 vertices).
+// Vertices per strand block for each strand length and launch-shape variant:
+// variant 0 = 512, 1 = 1024, 2 = 2048 guide vertices (at most 128 strands).
 extern "C" int tfx_fused_block_vertices(int V, int variant) {
     if (V != 4 && V != 8 && V != 16 && V != 32 && V != 64) return 0;
-    const int vb = variant == 1 ? 2048 : 1024;
-    return V == 4 ? vb / 2 : vb;
+    if (variant < 0 || variant > 2) return 0;
+    const int vb = 512 << variant;
+    return vb / V > 128 ? 128 * V : vb;
 }
 
 extern "C" hipError_t tfx_launch_fused(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
                                        int nitems, int grid, hipStream_t stream) {
     using namespace tfx;
-    if (variant == 1) {
-        switch (V) {
-            case 4: return launch_fused<4, 1024>(hairs, items, nitems, grid, stream);
-            case 8: return launch_fused<8, 2048>(hairs, items, nitems, grid, stream);
-            case 16: return launch_fused<16, 2048>(hairs, items, nitems, grid, stream);
-            case 32: return launch_fused<32, 2048>(hairs, items, nitems, grid, stream);
-            case 64: return launch_fused<64, 2048>(hairs, items, nitems, grid, stream);
-            default: return hipErrorInvalidValue;
-        }
-    }
-    switch (V) {
-        case 4: return launch_fused<4, 512>(hairs, items, nitems, grid, stream);
-        case 8: return launch_fused<8, 1024>(hairs, items, nitems, grid, stream);
-        case 16: return launch_fused<16, 1024>(hairs, items, nitems, grid, stream);
-        case 32: return launch_fused<32, 1024>(hairs, items, nitems, grid, stream);
-        case 64: return launch_fused<64, 1024>(hairs, items, nitems, grid, stream);
+    const int vb = tfx_fused_block_vertices(V, variant);
+    switch (V * 10000 + vb) {
+        case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, stream);
+        case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, stream);
+        case 8 * 10000 + 1024: return launch_fused<8, 1024>(hairs, items, nitems, grid, stream);
+        case 16 * 10000 + 512: return launch_fused<16, 512>(hairs, items, nitems, grid, stream);
+        case 16 * 10000 + 1024: return launch_fused<16, 1024>(hairs, items, nitems, grid, stream);
+        case 16 * 10000 + 2048: return launch_fused<16, 2048>(hairs, items, nitems, grid, stream);
+        case 32 * 10000 + 512: return launch_fused<32, 512>(hairs, items, nitems, grid, stream);
+        case 32 * 10000 + 1024: return launch_fused<32, 1024>(hairs, items, nitems, grid, stream);
+        case 32 * 10000 + 2048: return launch_fused<32, 2048>(hairs, items, nitems, grid, stream);
+        case 64 * 10000 + 512: return launch_fused<64, 512>(hairs, items, nitems, grid, stream);
+        case 64 * 10000 + 1024: return launch_fused<64, 1024>(hairs, items, nitems, grid, stream);
+        case 64 * 10000 + 2048: return launch_fused<64, 2048>(hairs, items, nitems, grid, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+// Resident blocks per CU of one launch shape (0 if the shape does not exist).
+extern "C" int tfx_fused_occupancy(int V, int variant) {
+    using namespace tfx;
+    const int vb = tfx_fused_block_vertices(V, variant);
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define TFX_OCC(Vv, VBv) \
+    if (V == Vv && vb == VBv) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_substep<Vv, VBv>, kFusedBlock, 0);
+    TFX_OCC(4, 512) TFX_OCC(8, 512) TFX_OCC(8, 1024) TFX_OCC(16, 512) TFX_OCC(16, 1024) TFX_OCC(16, 2048)
+    TFX_OCC(32, 512) TFX_OCC(32, 1024) TFX_OCC(32, 2048) TFX_OCC(64, 512) TFX_OCC(64, 1024) TFX_OCC(64, 2048)
+#undef TFX_OCC
+    return e == hipSuccess ? n : 0;
 }

@@ -79,16 +79,43 @@ __device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p) {
     return (dot3(sub(p, t.x0), t.nTri) < 0.0f) ? -d : d;
 }
 
-// InitializeSignedDistanceField :301-312
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_init(uint32_t* __restrict__ sdf, int ncell) {
+// InitializeSignedDistanceField :301-312, plus clearing the brick occupancy.
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_init(uint32_t* __restrict__ sdf, int ncell,
+                                                        uint8_t* __restrict__ bricks, int nbrick) {
     const uint32_t v = float_flip3(1e10f);
     for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock) sdf[i] = v;
+    if (bricks)
+        for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < nbrick; i += gridDim.x * kSdfBlock) bricks[i] = 0;
 }
 
-// FinalizeSignedDistanceField :400-412
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize(uint32_t* __restrict__ sdf, int ncell) {
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock)
-        sdf[i] = ifloat_flip3(sdf[i]);
+// FinalizeSignedDistanceField :400-412.  A cell that ends up closer than 1e10
+// also marks every brick whose kBrickHalo-dilated extent contains it.
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize(uint32_t* __restrict__ sdf, int ncell, SdfGrid g,
+                                                            uint8_t* __restrict__ bricks) {
+    const uint32_t far = float_flip3(1e10f);
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock) {
+        const uint32_t u = sdf[i];
+        sdf[i] = ifloat_flip3(u);
+        if (bricks && u != far) {
+            const int x = i % g.nx, r = i / g.nx, y = r % g.ny, z = r / g.ny;
+            const int x0 = max(x - kBrickHalo, 0) >> kBrickShift, x1 = min((x + kBrickHalo) >> kBrickShift, g.bx - 1);
+            const int y0 = max(y - kBrickHalo, 0) >> kBrickShift, y1 = min((y + kBrickHalo) >> kBrickShift, g.by - 1);
+            const int z0 = max(z - kBrickHalo, 0) >> kBrickShift, z1 = min((z + kBrickHalo) >> kBrickShift, g.bz - 1);
+            for (int bz = z0; bz <= z1; ++bz)
+                for (int by = y0; by <= y1; ++by)
+                    for (int bx = x0; bx <= x1; ++bx) bricks[(bz * g.by + by) * g.bx + bx] = 1;
+        }
+    }
+}
+
+// Brick occupancy packed to one bit per brick (word w covers bricks 32w..32w+31).
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_bricks(const uint8_t* __restrict__ bricks, int nbrick,
+                                                               uint32_t* __restrict__ bits) {
+    const int w = blockIdx.x * kSdfBlock + threadIdx.x;
+    if (w * 32 >= nbrick) return;
+    uint32_t m = 0;
+    for (int k = 0; k < 32 && w * 32 + k < nbrick; ++k) m |= (bricks[w * 32 + k] != 0 ? 1u : 0u) << k;
+    bits[w] = m;
 }
 
 // ConstructSignedDistanceField :329-397.  kTeam lanes per triangle; the team
@@ -152,13 +179,16 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_collide(float4* __restrict__ 
 
 }  // namespace tfx
 
-// phase 0 = Initialize, 1 = Construct, 2 = Finalize
+// phase 0 = Initialize, 1 = Construct, 2 = Finalize, 3 = pack the brick occupancy to bits
 extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
                                            const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream) {
     using namespace tfx;
     const int cblocks = std::min((ncell + kSdfBlock - 1) / kSdfBlock, 4096);
+    uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
+    const int nbrick = g.bx * g.by * g.bz;
     if (phase == 0) {
-        if (ncell > 0) hipLaunchKernelGGL(k_sdf_init, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
+        if (ncell > 0)
+            hipLaunchKernelGGL(k_sdf_init, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell, bricks, nbrick);
     } else if (phase == 1) {
         if (ntri > 0) {
             constexpr int kTeam = 16;
@@ -168,7 +198,13 @@ extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, 
                                g);
         }
     } else if (phase == 2) {
-        if (ncell > 0) hipLaunchKernelGGL(k_sdf_finalize, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell);
+        if (ncell > 0) hipLaunchKernelGGL(k_sdf_finalize, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell, g, bricks);
+    } else if (phase == 3) {
+        if (bricks && g.brick_bits) {
+            const int words = (nbrick + 31) / 32;
+            hipLaunchKernelGGL(k_sdf_pack_bricks, dim3((words + kSdfBlock - 1) / kSdfBlock), dim3(kSdfBlock), 0, stream,
+                               g.bricks, nbrick, const_cast<uint32_t*>(g.brick_bits));
+        }
     } else {
         return hipErrorInvalidValue;
     }
