@@ -305,6 +305,42 @@ int tfx_sdf_download(tfx_sdf* sdf, void* stream, float* host_cells /* X*Y*Z */);
 /* The constant block CollideWithHair would upload for `hair` (TressFXSDFCollision.cpp:184-198). */
 int tfx_sdf_get_collision_params(const tfx_sdf* sdf, const tfx_hair* hair, tfx_sdf_collision_params* out);
 
+/* ------------------------------------------------------------------------ */
+/* Batched form (no counterpart in the reference, whose Simulate dispatches  */
+/* every stage once per object, TressFXSimulation.cpp:51-75): many objects   */
+/* of uneven size advance in one persistent launch per substep and strand    */
+/* length, each colliding with its own SDF; SDFs rebuilt by one launch per   */
+/* build phase; the frame optionally captured into a hipGraph.               */
+/* Per frame: tfx_batch_begin_frame, tfx_batch_stage_substep per substep,    */
+/* tfx_batch_run_frame.  Objects are not internally synchronised: a member   */
+/* must not be used through the per-object API while a frame is in flight.  */
+/* ------------------------------------------------------------------------ */
+int tfx_batch_create(tfx_batch** out);
+int tfx_batch_destroy(tfx_batch* batch);
+/* Adds an object (and the SDF it collides with, or NULL). */
+int tfx_batch_add(tfx_batch* batch, tfx_hair* hair, tfx_sdf* sdf);
+int tfx_batch_remove(tfx_batch* batch, tfx_hair* hair);
+int tfx_batch_count(const tfx_batch* batch, int* num_members);
+/* As tfx_simulation_set_launch_shape, for the batch's fused launches. */
+int tfx_batch_set_launch_shape(tfx_batch* batch, int vertices_per_strand, int variant, int blocks_per_cu);
+/* Takes a free staging slot and moves every member's SDF grid to its mesh's
+ * current bounding box (meshes are set with tfx_sdf_set_mesh beforehand). */
+int tfx_batch_begin_frame(tfx_batch* batch);
+/* Stages one substep for every member, in member order: bone matrices
+ * (member m's at bones + m*bones_stride floats, bones_stride/16 matrices; NULL
+ * keeps the current ones), UpdateSimulationParameters with settings[m] (or
+ * settings[0] when num_settings == 1) and time_step, wind_magnitude[m]
+ * overriding the settings' magnitude (NULL: no override), then
+ * IncreaseSimulationFrame.  At most 16 substeps per frame. */
+int tfx_batch_stage_substep(tfx_batch* batch, const float* bones, int bones_stride,
+                            const tfx_simulation_settings* settings, int num_settings, float time_step,
+                            const float* wind_magnitude);
+/* Runs the staged frame on `stream`: one parameter upload, the SDF build of
+ * every member that has one, then the staged substeps (fused simulation +
+ * collision).  use_graph != 0: captured once per slot, substep count and
+ * membership, then replayed (needs a non-default stream). */
+int tfx_batch_run_frame(tfx_batch* batch, void* stream, int use_graph);
+
 #ifdef __cplusplus
 }
 #endif

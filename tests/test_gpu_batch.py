@@ -1,0 +1,168 @@
+"""The batched form (tfx_batch_*): objects of mixed strand length, some with
+an SDF collider, advanced together -- launched directly and replayed from a
+hipGraph -- must equal the per-object reference dispatch (Simulate, then
+CollideWithHair per object, src/Simulation.cpp:50-131) bit for bit, and the
+per-object path is itself pinned to the oracle by test_gpu_parity."""
+import numpy as np
+import pytest
+
+import tressfx_amd as tfx
+from tressfx_amd import synth
+
+from tfx_testutil import DeviceMesh, first_mismatch, set_substep_inputs
+from test_gpu_parity import make_hair, small_workload
+
+pytestmark = pytest.mark.gpu
+
+# (guides, V, follow per guide, SDF: None | ("explicit", n) | ("cells_x", n))
+MEMBERS = [(50, 16, 0, ("explicit", 64)), (300, 32, 1, ("cells_x", 40)), (70, 64, 0, None),
+           (33, 8, 2, ("explicit", 48)), (129, 4, 0, None), (211, 32, 0, ("explicit", 64))]
+CELL = 0.005
+
+
+class Member:
+    def __init__(self, torch, i, G, V, F, sdf_kind, copies):
+        self.wl = small_workload(G, V, F, seed=200 + i)
+        self.hairs = [make_hair(self.wl) for _ in range(copies)]
+        self.dm = None
+        self.sdfs = [None] * copies
+        if sdf_kind is not None:
+            rng = np.random.Generator(np.random.PCG64(300 + i))
+            mesh = synth.NoisySphereMesh(rng, radius=0.1, subdivisions=2 + (i % 2), amp=0.01)
+            self.dm = DeviceMesh(torch, mesh, synth.head_yaw)
+            desc = self.dm.update(0.0)
+            kind, n = sdf_kind
+            if kind == "explicit":
+                ex = ((-0.5 * n * CELL,) * 3, CELL, (n,) * 3)
+                self.sdfs = [tfx.TressFXSDFCollision(desc, collision_margin=3.0, explicit=ex) for _ in range(copies)]
+            else:
+                self.sdfs = [tfx.TressFXSDFCollision(desc, num_cells_in_x=n, collision_margin=1.0)
+                             for _ in range(copies)]
+
+    def substep_inputs(self, frame, sub):
+        wl = self.wl
+        t = (frame + (sub + 1) / wl.substeps) / synth.FRAME_RATE
+        bones = np.asarray(wl.bone_fn(t), np.float32).reshape(-1)
+        st = tfx.SimulationSettings.from_buffer_copy(wl.settings)
+        if wl.wind_fn is not None:
+            st.wind_magnitude = wl.wind_fn(t)
+        return bones, st
+
+
+def _run(torch, members, frames, substeps_of, plain_copy, batch_copies, stream_of, graph_of, batches=None):
+    """Advances copy `plain_copy` of every member through the per-object
+    reference dispatch and each batch in `batch_copies` through the batch
+    API (new batches unless given); compares after every frame."""
+    system = tfx.TressFXSDFCollisionSystem()
+    sim = tfx.TressFXSimulation()
+    if batches is None:
+        batches = {}
+        for c in batch_copies:
+            b = tfx.HairBatch()
+            for m in members:
+                b.add(m.hairs[c], m.sdfs[c])
+            assert len(b) == len(members)
+            batches[c] = b
+    for frame in frames:
+        nsub = substeps_of(frame)
+        for m in members:
+            if m.dm is not None:
+                desc = m.dm.update(frame / synth.FRAME_RATE)
+                for s in m.sdfs:
+                    s.SetMesh(desc)
+        torch.cuda.synchronize()
+        # per-object reference order
+        for m in members:
+            if m.sdfs[plain_copy] is not None:
+                m.sdfs[plain_copy].Update(system)
+        for sub in range(nsub):
+            for m in members:
+                m.wl.substeps = nsub
+                set_substep_inputs(m.hairs[plain_copy], m.wl, frame, sub)
+            sim.Simulate([m.hairs[plain_copy] for m in members])
+            for m in members:
+                if m.sdfs[plain_copy] is not None:
+                    m.sdfs[plain_copy].CollideWithHair(system, m.hairs[plain_copy])
+        # batches
+        for c, b in batches.items():
+            b.begin_frame()
+            for sub in range(nsub):
+                ins = [m.substep_inputs(frame, sub) for m in members]
+                bones = np.stack([x[0] for x in ins])
+                settings = (tfx.SimulationSettings * len(members))(*[x[1] for x in ins])
+                b.stage_substep(bones, settings, np.float32(1.0 / synth.FRAME_RATE / nsub))
+            b.run_frame(stream_of(c), use_graph=graph_of(c))
+        torch.cuda.synchronize()
+        for m in members:
+            ref = m.hairs[plain_copy].download_state()
+            for c in batches:
+                got = m.hairs[c].download_state()
+                for k in ("pos", "prev", "tangent"):
+                    mm = first_mismatch(got[k], ref[k])
+                    assert mm is None, (f"frame {frame} copy {c} V={m.wl.asset.num_vertices_per_strand} "
+                                        f"G={m.wl.asset.num_guide_strands} {k}: {mm}")
+                assert m.hairs[c].info().simulation_frame == m.hairs[plain_copy].info().simulation_frame
+    return batches
+
+
+def test_batch_direct_and_graph_equal_per_object_dispatch(gpu):
+    torch = gpu
+    print("[batch] direct + graph vs per-object, 6 members, 5 frames", flush=True)
+    members = [Member(torch, i, *spec, copies=3) for i, spec in enumerate(MEMBERS)]
+    side = torch.cuda.Stream()
+    _run(torch, members, range(5), lambda f: 2, plain_copy=0, batch_copies=(1, 2),
+         stream_of=lambda c: None if c == 1 else side.cuda_stream, graph_of=lambda c: c == 2)
+
+
+def test_batch_graph_recaptures_on_substep_change(gpu):
+    """A frame with another substep count re-captures its slot's graph; both
+    slots alternate between frames."""
+    torch = gpu
+    print("[batch] graph with 2,2,1,3,2,2 substeps", flush=True)
+    members = [Member(torch, i, *spec, copies=2) for i, spec in enumerate(MEMBERS[:4])]
+    side = torch.cuda.Stream()
+    counts = [2, 2, 1, 3, 2, 2]
+    _run(torch, members, range(len(counts)), lambda f: counts[f], plain_copy=0, batch_copies=(1,),
+         stream_of=lambda c: side.cuda_stream, graph_of=lambda c: True)
+
+
+def test_batch_membership_change(gpu):
+    """Removing and adding members re-lays out the batch; the remaining
+    members keep stepping exactly."""
+    torch = gpu
+    print("[batch] membership change", flush=True)
+    members = [Member(torch, i, *spec, copies=2) for i, spec in enumerate(MEMBERS[:4])]
+    side = torch.cuda.Stream()
+    batches = _run(torch, members, range(2), lambda f: 2, plain_copy=0, batch_copies=(1,),
+                   stream_of=lambda c: side.cuda_stream, graph_of=lambda c: True)
+    b = batches[1]
+    b.remove(members[1].hairs[1])
+    assert len(b) == 3
+    with pytest.raises(tfx.TFXError):
+        b.remove(members[1].hairs[1])
+    with pytest.raises(tfx.TFXError):
+        b.add(members[0].hairs[1], members[0].sdfs[1])
+    rest = [members[0], members[2], members[3]]
+    _run(torch, rest, range(2, 4), lambda f: 2, plain_copy=0, batch_copies=(1,),
+         stream_of=lambda c: side.cuda_stream, graph_of=lambda c: True, batches=batches)
+    # and back in, with another launch shape
+    b.add(members[1].hairs[1], members[1].sdfs[1])
+    b.set_launch_shape(32, 0)
+    # neither copy of the removed object advanced meanwhile (the wind depends
+    # on the frame counter, so the counters must agree as well as the states)
+    assert members[1].hairs[1].info().simulation_frame == members[1].hairs[0].info().simulation_frame
+    _run(torch, rest + [members[1]], range(4, 6), lambda f: 2, plain_copy=0, batch_copies=(1,),
+         stream_of=lambda c: side.cuda_stream, graph_of=lambda c: True, batches=batches)
+
+
+def test_batch_errors(gpu):
+    b = tfx.HairBatch()
+    st = tfx.SimulationSettings.defaults()
+    with pytest.raises(tfx.TFXError):
+        b.stage_substep(None, st, 0.01)          # no frame begun
+    with pytest.raises(tfx.TFXError):
+        b.run_frame()
+    b.begin_frame()
+    b.run_frame()                                # empty batch: nothing to do
+    with pytest.raises(tfx.TFXError):
+        b.set_launch_shape(12, 0)

@@ -271,3 +271,42 @@ def test_fused_multi_object_mixed_lengths(gpu, variant):
                 for k in ("pos", "prev", "tangent"):
                     mm = first_mismatch(sb[k], sa[k])
                     assert mm is None, f"V={wl.asset.num_vertices_per_strand} {k}: {mm}"
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_fused_sdf_launch_variants(gpu, variant):
+    """Fused simulation + SDF collision at every block size equals plain
+    Simulate + CollideWithHair, with follow strands (staged and not)."""
+    torch = gpu
+    print(f"[fused sdf variants] variant={variant}", flush=True)
+    system = tfx.TressFXSDFCollisionSystem()
+    sim_p = tfx.TressFXSimulation()
+    sim_f = tfx.TressFXSimulation()
+    objs = []
+    for i, (G, V, F) in enumerate([(200, 32, 1), (90, 16, 5), (40, 64, 2)]):
+        wl, dm = _mesh_setup(torch, G=G, V=V, F=F, seed=40 + i)
+        desc = dm.update(0.0)
+        origin, cell, dims = wl.sdf_grid
+        sdfs = [tfx.TressFXSDFCollision(desc, collision_margin=wl.collision_margin, explicit=(origin, cell, dims))
+                for _ in range(2)]
+        objs.append((wl, dm, sdfs, tfx.TressFXHairObject(wl.asset), tfx.TressFXHairObject(wl.asset)))
+        sim_f.SetLaunchShape(V, variant)
+    for frame in range(3):
+        for wl, dm, sdfs, a, b in objs:
+            desc = dm.update(frame / synth.FRAME_RATE)
+            for s in sdfs:
+                s.SetMesh(desc)
+                s.Update(system)
+        for sub in range(2):
+            for wl, dm, sdfs, a, b in objs:
+                set_substep_inputs(a, wl, frame, sub)
+                set_substep_inputs(b, wl, frame, sub)
+            sim_p.Simulate([o[3] for o in objs])
+            for wl, dm, sdfs, a, b in objs:
+                sdfs[0].CollideWithHair(system, a)
+            sim_f.SimulateFused([o[4] for o in objs], [o[2][1] for o in objs])
+            for wl, dm, sdfs, a, b in objs:
+                sa, sb = a.download_state(), b.download_state()
+                for k in ("pos", "prev", "tangent"):
+                    mm = first_mismatch(sb[k], sa[k])
+                    assert mm is None, f"V={wl.asset.num_vertices_per_strand} frame {frame} sub {sub} {k}: {mm}"

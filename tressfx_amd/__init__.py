@@ -220,6 +220,15 @@ EXPORTS = {
     "tfx_sdf_get_data_buffer": (_I, [_VP, ctypes.POINTER(_VP)]),
     "tfx_sdf_download": (_I, [_VP, _VP, c_float_p]),
     "tfx_sdf_get_collision_params": (_I, [_VP, _VP, ctypes.POINTER(SdfCollisionParams)]),
+    "tfx_batch_create": (_I, [ctypes.POINTER(_VP)]),
+    "tfx_batch_destroy": (_I, [_VP]),
+    "tfx_batch_add": (_I, [_VP, _VP, _VP]),
+    "tfx_batch_remove": (_I, [_VP, _VP]),
+    "tfx_batch_count": (_I, [_VP, ctypes.POINTER(_I)]),
+    "tfx_batch_set_launch_shape": (_I, [_VP, _I, _I, _I]),
+    "tfx_batch_begin_frame": (_I, [_VP]),
+    "tfx_batch_stage_substep": (_I, [_VP, c_float_p, _I, ctypes.POINTER(SimulationSettings), _I, _F, c_float_p]),
+    "tfx_batch_run_frame": (_I, [_VP, _VP, _I]),
 }
 
 _lib = None
@@ -519,3 +528,67 @@ class TressFXSDFCollision:
         out = np.zeros(g.num_cells_x * g.num_cells_y * g.num_cells_z, np.float32)
         _check(self._lib.tfx_sdf_download(self.handle, stream, _fptr(out)))
         return out
+
+
+class HairBatch:
+    """The batched form over many TressFXHairObject members (tfx_batch_*)."""
+
+    def __init__(self):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.tfx_batch_create(ctypes.byref(h)))
+        self.handle = h
+        self.members = []
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, hair, sdf=None):
+        _check(self._lib.tfx_batch_add(self.handle, hair.handle, sdf.handle if sdf is not None else None))
+        self.members.append((hair, sdf))
+
+    def remove(self, hair):
+        _check(self._lib.tfx_batch_remove(self.handle, hair.handle))
+        self.members = [m for m in self.members if m[0] is not hair]
+
+    def __len__(self):
+        n = ctypes.c_int()
+        _check(self._lib.tfx_batch_count(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def set_launch_shape(self, vertices_per_strand, variant, blocks_per_cu=0):
+        _check(self._lib.tfx_batch_set_launch_shape(self.handle, vertices_per_strand, variant, blocks_per_cu))
+
+    def begin_frame(self):
+        _check(self._lib.tfx_batch_begin_frame(self.handle))
+
+    def stage_substep(self, bones, settings, time_step, wind_magnitude=None):
+        """bones: float32 [members, k*16] (or None); settings: one
+        SimulationSettings or a ctypes array of one per member."""
+        bp, stride = None, 0
+        if bones is not None:
+            b = np.ascontiguousarray(bones, np.float32)
+            b = b.reshape(b.shape[0], -1)
+            bp, stride = _fptr(b), b.shape[1]
+            self._keep = b
+        if isinstance(settings, SimulationSettings):
+            sp, ns = ctypes.pointer(settings), 1
+        else:
+            sp, ns = ctypes.cast(settings, ctypes.POINTER(SimulationSettings)), len(settings)
+        wp = None
+        if wind_magnitude is not None:
+            w = np.ascontiguousarray(wind_magnitude, np.float32)
+            wp = _fptr(w)
+            self._keepw = w
+        _check(self._lib.tfx_batch_stage_substep(self.handle, bp, stride, sp, ns, ctypes.c_float(time_step), wp))
+
+    def run_frame(self, stream=None, use_graph=False):
+        _check(self._lib.tfx_batch_run_frame(self.handle, stream, 1 if use_graph else 0))

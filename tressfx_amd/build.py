@@ -16,6 +16,7 @@ SOURCES = [
     "tfx_sim_kernels.hip",
     "tfx_fused_kernel.hip",
     "tfx_sdf_kernels.hip",
+    "tfx_batch.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,353 @@
+// tfx_batch.hip — the batched form: many hair objects of uneven size advance
+// together.  Per substep, one persistent fused launch per strand-length class
+// covers every member through an (object, strand block) work list; per frame,
+// every member's SDF is rebuilt by one launch per build phase.  The frame —
+// one parameter upload, the SDF build and all substeps — can be captured once
+// into a hipGraph and replayed while membership is unchanged.
+//
+// The per-frame order of the reference (src/Simulation.cpp:50-131) is kept
+// for every member: SDF Update, then per substep Simulate followed by
+// CollideWithHair (each member collides with its own SDF).
+#include "tfx_internal.h"
+
+extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
+                                           hipStream_t stream);
+
+namespace {
+
+constexpr int kBatchMaxSubsteps = 16;
+
+struct Member {
+    tfx_hair* hair;
+    tfx_sdf* sdf;
+};
+
+}  // namespace
+
+struct tfx_batch {
+    int device = 0;
+    std::vector<Member> members;
+    bool dirty = true;                 // membership changed: tables, work lists and graphs are stale
+    int cap = 0;                       // substep slots laid out
+    tfx_simulation shape;              // launch-shape policy (variants per strand length)
+
+    // Slot layout (host pinned slot and its device copy are identical):
+    //   DevParams[cap][n] | bones float[cap][bone_floats] | SdfJob[nsdf]
+    size_t off_bones = 0, off_jobs = 0, slot_bytes = 0;
+    std::vector<int> bone_off;         // per member, floats into one substep's bone block
+    int bone_floats = 0;
+    std::vector<int> sdf_members;      // member indices that own an SDF
+    int max_cells = 0, max_tris = 0;
+
+    void* dev_slot = nullptr;          // device copy of a slot
+    DevHair* dev_tables = nullptr;     // [cap][n]
+    tfx::WorkItem* dev_items = nullptr;
+    int vbegin[65] = {0}, vcount[65] = {0}, vvariant[65] = {0}, vresident[65] = {0};
+
+    void* host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t host_bytes = 0;
+    int slot = -1;                     // slot being staged, -1 = none
+    int next_slot = 0;
+    int staged = 0;                    // substeps staged into `slot`
+
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int exec_substeps[2] = {0, 0};     // substeps captured in exec[k]
+    int exec_tris[2] = {0, 0};         // SDF construct grid (largest mesh) captured in exec[k]
+};
+
+static void release_device(tfx_batch* b) {
+    (void)hipFree(b->dev_slot);
+    (void)hipFree(b->dev_tables);
+    (void)hipFree(b->dev_items);
+    b->dev_slot = nullptr;
+    b->dev_tables = nullptr;
+    b->dev_items = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        b->exec[k] = nullptr;
+        b->exec_substeps[k] = 0;
+        b->exec_tris[k] = 0;
+    }
+}
+
+static void release_host(tfx_batch* b) {
+    for (int k = 0; k < 2; ++k) {
+        if (b->ev[k]) { (void)hipEventSynchronize(b->ev[k]); (void)hipEventDestroy(b->ev[k]); b->ev[k] = nullptr; }
+        if (b->host[k]) { (void)hipHostFree(b->host[k]); b->host[k] = nullptr; }
+    }
+    b->host_bytes = 0;
+}
+
+// Lays out slots, tables and work lists for the current membership.
+static int prepare(tfx_batch* b) {
+    if (!b->dirty) return TFX_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    release_device(b);
+    const int n = (int)b->members.size();
+    b->cap = kBatchMaxSubsteps;
+    b->bone_off.assign(n, 0);
+    b->bone_floats = 0;
+    b->sdf_members.clear();
+    b->max_cells = b->max_tris = 0;
+    for (int m = 0; m < n; ++m) {
+        b->bone_off[m] = b->bone_floats;
+        b->bone_floats += 16 * b->members[m].hair->bones_used;
+        if (tfx_sdf* s = b->members[m].sdf) {
+            b->sdf_members.push_back(m);
+            b->max_cells = std::max(b->max_cells, s->nx * s->ny * s->nz);
+        }
+    }
+    const size_t params_bytes = (size_t)b->cap * n * sizeof(DevParams);
+    b->off_bones = (params_bytes + 255) & ~size_t(255);
+    const size_t bones_bytes = (size_t)b->cap * b->bone_floats * sizeof(float);
+    b->off_jobs = (b->off_bones + bones_bytes + 255) & ~size_t(255);
+    b->slot_bytes = b->off_jobs + b->sdf_members.size() * sizeof(tfx::SdfJob);
+    if (b->slot_bytes > b->host_bytes) {
+        release_host(b);
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(hipHostMalloc(&b->host[k], b->slot_bytes, hipHostMallocDefault));
+            HIP_TRY(hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming));
+        }
+        b->host_bytes = b->slot_bytes;
+    }
+    if (n > 0) HIP_TRY(hipMalloc(&b->dev_slot, b->slot_bytes));
+    // DevHair tables: member m at substep s reads its parameters and bones from slot s.
+    std::vector<DevHair> tables((size_t)b->cap * n);
+    for (int s = 0; s < b->cap; ++s)
+        for (int m = 0; m < n; ++m) {
+            DevHair d = dev_hair(b->members[m].hair, b->members[m].sdf);
+            d.params = reinterpret_cast<const DevParams*>(b->dev_slot) + (size_t)s * n + m;
+            d.bones = reinterpret_cast<const float*>((char*)b->dev_slot + b->off_bones) +
+                      (size_t)s * b->bone_floats + b->bone_off[m];
+            tables[(size_t)s * n + m] = d;
+        }
+    if (n > 0) {
+        HIP_TRY(hipMalloc((void**)&b->dev_tables, tables.size() * sizeof(DevHair)));
+        HIP_TRY(hipMemcpy(b->dev_tables, tables.data(), tables.size() * sizeof(DevHair), hipMemcpyHostToDevice));
+    }
+    // Work lists per strand-length class (identical for every substep).
+    std::vector<tfx_hair*> hairs(n);
+    for (int m = 0; m < n; ++m) hairs[m] = b->members[m].hair;
+    std::vector<tfx::WorkItem> items;
+    const int cus = cu_count(b->device);
+    for (int V = 4; V <= 64; V *= 2) {
+        b->vbegin[V] = (int)items.size();
+        b->vcount[V] = 0;
+        long long guides = 0;
+        for (int m = 0; m < n; ++m)
+            if (hairs[m]->V == V) guides += hairs[m]->G;
+        if (!guides) continue;
+        const Shape sh = choose_shape(&b->shape, V, hairs.data(), n, cus);
+        b->vvariant[V] = sh.variant;
+        b->vresident[V] = sh.resident;
+        const int B = tfx_fused_block_vertices(V, sh.variant) / V;
+        // Large objects first: their blocks start early and the tail is made
+        // of the small ones.
+        std::vector<int> order;
+        for (int m = 0; m < n; ++m)
+            if (hairs[m]->V == V) order.push_back(m);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return hairs[x]->G > hairs[y]->G; });
+        for (int m : order)
+            for (int g0 = 0; g0 < hairs[m]->G; g0 += B) items.push_back(tfx::WorkItem{m, g0});
+        b->vcount[V] = (int)items.size() - b->vbegin[V];
+    }
+    if (!items.empty()) {
+        HIP_TRY(hipMalloc((void**)&b->dev_items, items.size() * sizeof(tfx::WorkItem)));
+        HIP_TRY(hipMemcpy(b->dev_items, items.data(), items.size() * sizeof(tfx::WorkItem), hipMemcpyHostToDevice));
+    }
+    b->dirty = false;
+    return TFX_OK;
+}
+
+extern "C" int tfx_batch_create(tfx_batch** out) {
+    CHECK_ARG(out, "tfx_batch_create: null argument");
+    tfx_batch* b = new tfx_batch();
+    b->device = g_device;
+    b->shape.device = g_device;
+    for (int v = 0; v <= 64; ++v) b->shape.variant[v] = -1;
+    *out = b;
+    return TFX_OK;
+}
+
+extern "C" int tfx_batch_destroy(tfx_batch* b) {
+    if (!b) return TFX_OK;
+    (void)hipDeviceSynchronize();
+    release_device(b);
+    release_host(b);
+    delete b;
+    return TFX_OK;
+}
+
+extern "C" int tfx_batch_add(tfx_batch* b, tfx_hair* hair, tfx_sdf* sdf) {
+    CHECK_ARG(b && hair, "tfx_batch_add: null argument");
+    CHECK_ARG(b->slot < 0, "tfx_batch_add: a frame is being staged");
+    for (const Member& m : b->members)
+        if (m.hair == hair) return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_batch_add: hair already a member");
+    b->members.push_back(Member{hair, sdf});
+    b->dirty = true;
+    return TFX_OK;
+}
+
+extern "C" int tfx_batch_remove(tfx_batch* b, tfx_hair* hair) {
+    CHECK_ARG(b && hair, "tfx_batch_remove: null argument");
+    CHECK_ARG(b->slot < 0, "tfx_batch_remove: a frame is being staged");
+    for (size_t i = 0; i < b->members.size(); ++i)
+        if (b->members[i].hair == hair) {
+            b->members.erase(b->members.begin() + i);
+            b->dirty = true;
+            return TFX_OK;
+        }
+    return fail(TFX_ERR_NOT_FOUND, "tfx_batch_remove: not a member");
+}
+
+extern "C" int tfx_batch_count(const tfx_batch* b, int* n) {
+    CHECK_ARG(b && n, "tfx_batch_count: null argument");
+    *n = (int)b->members.size();
+    return TFX_OK;
+}
+
+extern "C" int tfx_batch_set_launch_shape(tfx_batch* b, int V, int variant, int blocks_per_cu) {
+    CHECK_ARG(b, "tfx_batch_set_launch_shape: null batch");
+    int rc = tfx_simulation_set_launch_shape(&b->shape, V, variant, blocks_per_cu);
+    if (rc) return rc;
+    b->dirty = true;
+    return TFX_OK;
+}
+
+// Starts a frame: takes a free staging slot and moves every member's SDF grid
+// to its mesh's current bounding box (TressFXSDFCollision::UpdateSDFGrid,
+// TressFXSDFCollision.cpp:88-93).
+extern "C" int tfx_batch_begin_frame(tfx_batch* b) {
+    CHECK_ARG(b, "tfx_batch_begin_frame: null batch");
+    CHECK_ARG(b->slot < 0, "tfx_batch_begin_frame: previous frame not run");
+    int rc = prepare(b);
+    if (rc) return rc;
+    const int k = b->next_slot;
+    b->next_slot ^= 1;
+    if (b->ev[k]) HIP_TRY(hipEventSynchronize(b->ev[k]));
+    b->slot = k;
+    b->staged = 0;
+    tfx::SdfJob* jobs = reinterpret_cast<tfx::SdfJob*>((char*)b->host[k] + b->off_jobs);
+    b->max_tris = 0;
+    for (size_t j = 0; j < b->sdf_members.size(); ++j) {
+        tfx_sdf* s = b->members[b->sdf_members[j]].sdf;
+        if (!s->explicit_grid)
+            for (int a = 0; a < 3; ++a) s->origin[a] = s->mesh.bbox_min[a] - s->pad_boundary;
+        tfx::SdfJob& jb = jobs[j];
+        jb.cells = s->cells;
+        jb.verts = s->mesh.vertices;
+        jb.idx = s->mesh.indices;
+        jb.ntri = (s->mesh.vertices && s->mesh.indices) ? s->mesh.num_triangles : 0;
+        jb.ncell = s->nx * s->ny * s->nz;
+        jb.g = sdf_grid(s);
+        b->max_tris = std::max(b->max_tris, jb.ntri);
+    }
+    return TFX_OK;
+}
+
+// Stages one substep for every member: bone matrices and
+// UpdateSimulationParameters (TressFXHairObject.cpp:366-437) per member, the
+// packed block copied into the slot, then IncreaseSimulationFrame.
+//   bones: member m's matrices at bones + m * bones_stride (floats), up to its bones_used
+//   settings: one per member (n_settings == member count) or one for all
+//   wind_magnitude: per member override of settings.wind_magnitude, or NULL
+extern "C" int tfx_batch_stage_substep(tfx_batch* b, const float* bones, int bones_stride,
+                                       const tfx_simulation_settings* settings, int n_settings, float time_step,
+                                       const float* wind_magnitude) {
+    CHECK_ARG(b && settings, "tfx_batch_stage_substep: null argument");
+    CHECK_ARG(b->slot >= 0, "tfx_batch_stage_substep: call tfx_batch_begin_frame first");
+    CHECK_ARG(b->staged < b->cap, "tfx_batch_stage_substep: too many substeps in one frame");
+    const int n = (int)b->members.size();
+    CHECK_ARG(n_settings == 1 || n_settings == n, "tfx_batch_stage_substep: settings count");
+    CHECK_ARG(!bones || bones_stride >= 16, "tfx_batch_stage_substep: bones stride below one matrix");
+    char* host = (char*)b->host[b->slot];
+    DevParams* params = reinterpret_cast<DevParams*>(host) + (size_t)b->staged * n;
+    float* bone_block = reinterpret_cast<float*>(host + b->off_bones) + (size_t)b->staged * b->bone_floats;
+    for (int m = 0; m < n; ++m) {
+        tfx_hair* h = b->members[m].hair;
+        if (bones) {
+            const int nb = std::min(bones_stride / 16, TFX_MAX_NUM_BONES);
+            int rc = tfx_hair_update_bone_matrices(h, bones + (size_t)m * bones_stride, nb);
+            if (rc) return rc;
+        }
+        tfx_simulation_settings st = settings[n_settings == 1 ? 0 : m];
+        if (wind_magnitude) st.wind_magnitude = wind_magnitude[m];
+        int rc = tfx_hair_update_simulation_parameters(h, &st, time_step);
+        if (rc) return rc;
+        pack_dev_params(h, b->members[m].sdf, params[m]);
+        std::memcpy(bone_block + b->bone_off[m], h->bones_h.data(), (size_t)16 * h->bones_used * sizeof(float));
+        h->bones_dirty = TFX_MAX_NUM_BONES;  // the object's own table is stale for the plain path
+        tfx_hair_increase_simulation_frame(h);
+    }
+    b->staged++;
+    return TFX_OK;
+}
+
+static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
+    const int n = (int)b->members.size();
+    HIP_TRY(hipMemcpyAsync(b->dev_slot, b->host[b->slot], b->slot_bytes, hipMemcpyHostToDevice, s));
+    const int nsdf = (int)b->sdf_members.size();
+    if (nsdf) {
+        const tfx::SdfJob* jobs = reinterpret_cast<const tfx::SdfJob*>((char*)b->dev_slot + b->off_jobs);
+        static const char* kNames[4] = {"InitializeSignedDistanceField", "ConstructSignedDistanceField",
+                                        "FinalizeSignedDistanceField", "FinalizeSignedDistanceField"};
+        for (int phase = 0; phase < 4; ++phase) {
+            PassTimer tm(timers ? kNames[phase] : nullptr, s);
+            HIP_TRY(tfx_launch_sdf_multi(phase, jobs, nsdf, b->max_cells, b->max_tris, s));
+        }
+    }
+    for (int sub = 0; sub < substeps; ++sub)
+        for (int V = 4; V <= 64; V *= 2) {
+            if (!b->vcount[V]) continue;
+            const int grid = std::min(b->vcount[V], b->vresident[V]);
+            PassTimer tm(timers ? "FusedSubstep" : nullptr, s);
+            HIP_TRY(tfx_launch_fused(V, b->vvariant[V], b->dev_tables + (size_t)sub * n, b->dev_items + b->vbegin[V],
+                                     b->vcount[V], grid, s));
+        }
+    return TFX_OK;
+}
+
+// Runs the staged frame: parameter upload, batched SDF build, and the staged
+// substeps.  use_graph: the launches are captured once per (slot, substep
+// count, membership) into a hipGraph and replayed.
+extern "C" int tfx_batch_run_frame(tfx_batch* b, void* stream, int use_graph) {
+    CHECK_ARG(b, "tfx_batch_run_frame: null batch");
+    CHECK_ARG(b->slot >= 0, "tfx_batch_run_frame: no frame staged");
+    hipStream_t s = (hipStream_t)stream;
+    const int k = b->slot;
+    const int substeps = b->staged;
+    int rc = TFX_OK;
+    if (b->members.empty()) {
+        b->slot = -1;
+        return TFX_OK;
+    }
+    if (use_graph) {
+        CHECK_ARG(s != nullptr, "tfx_batch_run_frame: graph replay needs a non-default stream");
+        if (!b->exec[k] || b->exec_substeps[k] != substeps || b->exec_tris[k] != b->max_tris) {
+            if (b->exec[k]) { (void)hipGraphExecDestroy(b->exec[k]); b->exec[k] = nullptr; }
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            rc = launch_frame(b, s, substeps, 0);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); b->slot = -1; return rc; }
+            if (e != hipSuccess) { b->slot = -1; return fail(TFX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e)); }
+            e = hipGraphInstantiate(&b->exec[k], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) { b->slot = -1; return fail(TFX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+            b->exec_substeps[k] = substeps;
+            b->exec_tris[k] = b->max_tris;
+        }
+        PassTimer tm("BatchFrame", s);
+        hipError_t e = hipGraphLaunch(b->exec[k], s);
+        if (e != hipSuccess) { b->slot = -1; return fail(TFX_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e)); }
+    } else {
+        PassTimer tm("BatchFrame", s);
+        rc = launch_frame(b, s, substeps, 1);
+    }
+    hipError_t e = hipEventRecord(b->ev[k], s);
+    b->slot = -1;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TFX_ERR_HIP, "hipEventRecord: %s", hipGetErrorString(e));
+    return TFX_OK;
+}

@@ -289,6 +289,16 @@ struct SdfGrid {
 constexpr int kBrickShift = 2;  // 4^3 cells per brick
 constexpr int kBrickHalo = 2;
 
+// One SDF of a batched build: its cells, mesh and grid.
+struct SdfJob {
+    uint32_t* cells;
+    const tfx_standard_vertex* verts;
+    const uint32_t* idx;
+    int ntri;
+    int ncell;
+    SdfGrid g;
+};
+
 // GetSdfCoordinates :103-113
 __device__ __forceinline__ void sdf_coords(const SdfGrid& g, f3 p, int& cx, int& cy, int& cz) {
     cx = ftoi((p.x - g.ox) / g.cell);

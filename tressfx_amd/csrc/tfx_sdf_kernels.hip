@@ -219,3 +219,114 @@ extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, i
     hipLaunchKernelGGL(k_sdf_collide, dim3(blocks), dim3(kSdfBlock), 0, stream, pos, prev, n, V, sdf, g, margin);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Batched build: one launch per phase for many SDFs.  blockIdx.y selects the
+// job; the job table lives in device memory so that a captured graph replays
+// with new meshes and grid origins.
+// ---------------------------------------------------------------------------
+namespace tfx {
+
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_init_multi(const SdfJob* __restrict__ jobs) {
+    const SdfJob& j = jobs[blockIdx.y];
+    const uint32_t v = float_flip3(1e10f);
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < j.ncell; i += gridDim.x * kSdfBlock) j.cells[i] = v;
+    const int nbrick = j.g.bx * j.g.by * j.g.bz;
+    uint8_t* bricks = const_cast<uint8_t*>(j.g.bricks);
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < nbrick; i += gridDim.x * kSdfBlock) bricks[i] = 0;
+}
+
+template <int kTeam>
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob* __restrict__ jobs) {
+    const SdfJob& j = jobs[blockIdx.y];
+    const SdfGrid g = j.g;
+    const int lane = threadIdx.x % kTeam;
+    for (int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam; tri < j.ntri; tri += gridDim.x * (kSdfBlock / kTeam)) {
+        const uint32_t i0 = j.idx[3 * tri + 0], i1 = j.idx[3 * tri + 1], i2 = j.idx[3 * tri + 2];
+        const f3 x0{j.verts[i0].position[0], j.verts[i0].position[1], j.verts[i0].position[2]};
+        const f3 x1{j.verts[i1].position[0], j.verts[i1].position[1], j.verts[i1].position[2]};
+        const f3 x2{j.verts[i2].position[0], j.verts[i2].position[1], j.verts[i2].position[2]};
+        const float m = g.cell;
+        f3 mn{fmn(x0.x, fmn(x1.x, x2.x)) - m, fmn(x0.y, fmn(x1.y, x2.y)) - m, fmn(x0.z, fmn(x1.z, x2.z)) - m};
+        f3 mx{fmx(x0.x, fmx(x1.x, x2.x)) + m, fmx(x0.y, fmx(x1.y, x2.y)) + m, fmx(x0.z, fmx(x1.z, x2.z)) + m};
+        int lo[3], hi[3];
+        sdf_coords(g, mn, lo[0], lo[1], lo[2]);
+        sdf_coords(g, mx, hi[0], hi[1], hi[2]);
+        const int dims[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = (int)((uint32_t)lo[k] - 1u);
+            hi[k] = (int)((uint32_t)hi[k] + 1u);
+            lo[k] = max(0, min(lo[k], dims[k] - 1));
+            hi[k] = max(0, min(hi[k], dims[k] - 1));
+        }
+        const TriPrep tp = tri_prep(x0, x1, x2);
+        const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+        const int total = ex * ey * ez;
+        for (int c = lane; c < total; c += kTeam) {
+            const int xx = c % ex;
+            const int rem = c / ex;
+            const int yy = rem % ey;
+            const int zz = rem / ey;
+            const int x = lo[0] + xx, y = lo[1] + yy, z = lo[2] + zz;
+            const float d = signed_distance(tp, sdf_cell_pos(g, x, y, z));
+            atomicMin(&j.cells[sdf_index(g, x, y, z)], float_flip3(d));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* __restrict__ jobs) {
+    const SdfJob& j = jobs[blockIdx.y];
+    const SdfGrid g = j.g;
+    const uint32_t far = float_flip3(1e10f);
+    uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < j.ncell; i += gridDim.x * kSdfBlock) {
+        const uint32_t u = j.cells[i];
+        j.cells[i] = ifloat_flip3(u);
+        if (u != far) {
+            const int x = i % g.nx, r = i / g.nx, y = r % g.ny, z = r / g.ny;
+            const int x0 = max(x - kBrickHalo, 0) >> kBrickShift, x1 = min((x + kBrickHalo) >> kBrickShift, g.bx - 1);
+            const int y0 = max(y - kBrickHalo, 0) >> kBrickShift, y1 = min((y + kBrickHalo) >> kBrickShift, g.by - 1);
+            const int z0 = max(z - kBrickHalo, 0) >> kBrickShift, z1 = min((z + kBrickHalo) >> kBrickShift, g.bz - 1);
+            for (int bz = z0; bz <= z1; ++bz)
+                for (int by = y0; by <= y1; ++by)
+                    for (int bx = x0; bx <= x1; ++bx) bricks[(bz * g.by + by) * g.bx + bx] = 1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_multi(const SdfJob* __restrict__ jobs) {
+    const SdfJob& j = jobs[blockIdx.y];
+    const int nbrick = j.g.bx * j.g.by * j.g.bz;
+    uint32_t* bits = const_cast<uint32_t*>(j.g.brick_bits);
+    for (int w = blockIdx.x * kSdfBlock + threadIdx.x; w * 32 < nbrick; w += gridDim.x * kSdfBlock) {
+        uint32_t m = 0;
+        for (int k = 0; k < 32 && w * 32 + k < nbrick; ++k) m |= (j.g.bricks[w * 32 + k] != 0 ? 1u : 0u) << k;
+        bits[w] = m;
+    }
+}
+
+}  // namespace tfx
+
+// Batched build of `njobs` SDFs (job table in device memory).  `max_cells` /
+// `max_tris` bound the per-job work and size the grid's x extent.
+extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
+                                           hipStream_t stream) {
+    using namespace tfx;
+    if (njobs <= 0) return hipSuccess;
+    const int cx = std::max(1, std::min((max_cells + kSdfBlock - 1) / kSdfBlock, 1024));
+    switch (phase) {
+        case 0: hipLaunchKernelGGL(k_sdf_init_multi, dim3(cx, njobs), dim3(kSdfBlock), 0, stream, jobs); break;
+        case 1: {
+            constexpr int kTeam = 16;
+            const int tx = std::max(1, std::min((max_tris * kTeam + kSdfBlock - 1) / kSdfBlock, 4096));
+            hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kSdfBlock), 0, stream, jobs);
+            break;
+        }
+        case 2: hipLaunchKernelGGL(k_sdf_finalize_multi, dim3(cx, njobs), dim3(kSdfBlock), 0, stream, jobs); break;
+        case 3: hipLaunchKernelGGL(k_sdf_pack_multi, dim3(std::max(1, cx / 32), njobs), dim3(kSdfBlock), 0, stream, jobs);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
