@@ -11,9 +11,12 @@ Inputs (asset, per-frame collision meshes) are resident in HBM before the timed
 region; the per-substep parameter block and bone matrix are uploaded by the
 library, as the reference uploads its constant buffer.
 
-Printed line: one JSON object with value = vertex-substeps/s, the roofline of
-the dominant kernel (the fused substep, HBM-bound), the plain per-instance
-dispatch for comparison and the CPU transcription's throughput.
+The timed region replays the frame as a hipGraph through the batched form
+(one member).  Printed line: one JSON object with value = vertex-substeps/s,
+the roofline of the dominant kernel (the fused substep; its duration from HIP
+events around each launch in a second, directly launched leg), the plain
+per-instance dispatch for comparison, the 256-instance batched workload
+against its plain dispatch, and the CPU transcription's throughput.
 """
 import argparse
 import json
@@ -83,16 +86,43 @@ def cpu_baseline(wl, frames, nthreads, budget_s):
     return a.num_total_vertices * done_substeps / el, done_substeps, frame + 1, el
 
 
+def _timed_frames(torch, stream, drv, frames, **kw):
+    """Runs `frames` on `stream` between two device synchronisations; returns
+    (wall seconds, GPU ms per frame from events recorded between frames)."""
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(frames) + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for k, f in enumerate(frames):
+        drv.frame(f, stream.cuda_stream, **kw)
+        evs[k + 1].record(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    import numpy as np
+    return t1 - t0, np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(len(frames))])
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=600, help="timed frames of the config-2 workload")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--batched-steps", type=int, default=20, help="timed frames of the 256-instance workload")
+    ap.add_argument("--batched-plain-steps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=600)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--guides", type=int, default=20000)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="after the legs, copy a device buffer of this many bytes 3 times (a known-size "
+                         "kernel that calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU per process; multi-GPU runs are not part of this build "
@@ -100,53 +130,67 @@ def main():
         sys.exit(2)
 
     import numpy as np
+    from tressfx_amd import synth
+
+    host_workers = max(1, min(16, os.cpu_count() or 1))
+    # Workloads are generated before the process touches the GPU (the
+    # batched one by forked workers).
+    wl = synth.config2(num_guides=args.guides)
+    wls3 = None
+    if not args.no_batched:
+        _log(f"generating the 256-instance workload with {host_workers} processes")
+        wls3 = synth.config3(workers=host_workers)
+
     import torch
     import tressfx_amd as tfx
-    from tressfx_amd import harness, synth
+    from tressfx_amd import harness
 
     tfx.init(0)
     torch.cuda.init()
     stream = torch.cuda.Stream()
-    sh = ctypes_stream = stream.cuda_stream
 
-    wl = synth.config2(num_guides=args.guides)
     a = wl.asset
     G, F, V = a.num_guide_strands, a.num_follow_strands_per_guide, a.num_vertices_per_strand
     N = a.num_total_vertices
     W, K = args.warmup, args.steps
     frames = list(range(W + K))
-    drv = harness.FrameDriver(torch, wl, frames, fused=True)
-    print(f"[bench] config2 G={G} F={F} V={V} N={N} substeps={wl.substeps} warmup={W} steps={K}",
-          file=sys.stderr, flush=True)
+    drv = harness.BatchDriver(torch, [wl], frames)
+    _log(f"config2 G={G} F={F} V={V} N={N} substeps={wl.substeps} warmup={W} steps={K}")
 
     with torch.cuda.stream(stream):
         for f in range(W):
-            drv.frame(f, sh)
+            drv.frame(f, stream.cuda_stream, use_graph=True)
+        stream.synchronize()
+        elapsed, frame_ms = _timed_frames(torch, stream, drv, frames[W:], use_graph=True)
+        _log(f"graph leg {elapsed * 1e3 / K:.3f} ms/frame")
+    st = drv.hairs[0].download_state()
+    assert np.isfinite(st["pos"]).all(), "non-finite positions"
+    value = N * wl.substeps * K / elapsed
+
+    # Kernel leg: a second object runs the same frames from the same initial
+    # state, launched directly with a HIP event pair around every pass on the
+    # launch stream, so the fused kernel's average duration covers the same
+    # launches (same states, same per-launch work) as the timed region.
+    labels = ("FusedSubstep", "InitializeSignedDistanceField", "ConstructSignedDistanceField",
+              "FinalizeSignedDistanceField", "BatchFrame")
+    kdrv = harness.BatchDriver(torch, [wl], frames, meshes=drv.meshes)
+    with torch.cuda.stream(stream):
+        for f in range(W):
+            kdrv.frame(f, stream.cuda_stream, use_graph=False)
         stream.synchronize()
         tfx.profiling_enable(True)
-        for lbl in ("FusedSubstep", "InitializeSignedDistanceField", "ConstructSignedDistanceField",
-                    "FinalizeSignedDistanceField"):
+        for lbl in labels:
             tfx.profiling_read(lbl)
-        evs = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        evs[0].record(stream)
-        for k in range(K):
-            drv.frame(W + k, sh)
-            evs[k + 1].record(stream)
-        stream.synchronize()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+        _timed_frames(torch, stream, kdrv, frames[W:], use_graph=False)
         tfx.profiling_enable(False)
-    elapsed = t1 - t0
-    fused_ms, fused_n = tfx.profiling_read("FusedSubstep")
-    sdf_ms = sum(tfx.profiling_read(l)[0] for l in ("InitializeSignedDistanceField", "ConstructSignedDistanceField",
-                                                     "FinalizeSignedDistanceField"))
-    frame_ms = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])
-    value = N * wl.substeps * K / elapsed
-    st = drv.hair.download_state()
-    assert np.isfinite(st["pos"]).all(), "non-finite positions"
+        timers = {lbl: tfx.profiling_read(lbl) for lbl in labels}
+    kst = kdrv.hairs[0].download_state()
+    graph_equals_direct = bool(all(np.array_equal(st[k].view(np.uint32), kst[k].view(np.uint32))
+                                   for k in ("pos", "prev", "tangent")))
+    del drv, kdrv
 
+    fused_ms, fused_n = timers["FusedSubstep"]
+    sdf_ms = sum(timers[l][0] for l in labels[1:4])
     bytes_sub = algorithmic_bytes_per_substep(G, F, V)
     avg_s = fused_ms / max(fused_n, 1) / 1e3
     achieved = bytes_sub / avg_s / 1e9
@@ -159,23 +203,54 @@ def main():
         pdrv = harness.FrameDriver(torch, wl, pframes, fused=False)
         with torch.cuda.stream(stream):
             for f in range(W):
-                pdrv.frame(f, sh)
-            stream.synchronize()
-            p0 = time.perf_counter()
-            for k in range(args.plain_steps):
-                pdrv.frame(W + k, sh)
-            stream.synchronize()
-            p1 = time.perf_counter()
-        pv = N * wl.substeps * args.plain_steps / (p1 - p0)
-        plain = {"value": pv, "ms_per_step": (p1 - p0) * 1e3 / args.plain_steps, "steps": args.plain_steps}
+                pdrv.frame(f, stream.cuda_stream)
+            pel, _ = _timed_frames(torch, stream, pdrv, pframes[W:])
+        plain = {"value": N * wl.substeps * args.plain_steps / pel, "ms_per_step": pel * 1e3 / args.plain_steps,
+                 "steps": args.plain_steps}
+        _log(f"plain leg {plain['ms_per_step']:.3f} ms/frame")
         del pdrv
+
+    batched = None
+    if wls3 is not None:
+        n3 = sum(w.asset.num_total_vertices for w in wls3)
+        W3, K3, P3 = 3, args.batched_steps, args.batched_plain_steps
+        f3 = list(range(W3 + K3))
+        bdrv = harness.BatchDriver(torch, wls3, f3)
+        with torch.cuda.stream(stream):
+            for f in range(W3):
+                bdrv.frame(f, stream.cuda_stream, use_graph=True)
+            bel, bms = _timed_frames(torch, stream, bdrv, f3[W3:], use_graph=True)
+        _log(f"batched leg {bel * 1e3 / K3:.3f} ms/frame")
+        bval = n3 * 2 * K3 / bel
+        del bdrv
+        pval = None
+        if P3 > 0:
+            pf3 = list(range(1 + P3))
+            pdrv = harness.PlainMultiDriver(torch, wls3, pf3)
+            with torch.cuda.stream(stream):
+                pdrv.frame(0, stream.cuda_stream)
+                pel, _ = _timed_frames(torch, stream, pdrv, pf3[1:])
+            pval = n3 * 2 * P3 / pel
+            _log(f"batched-plain leg {pel * 1e3 / P3:.3f} ms/frame")
+            del pdrv
+        lengths = {v: sum(w.asset.num_guide_strands for w in wls3 if w.asset.num_vertices_per_strand == v)
+                   for v in (16, 32, 64)}
+        batched = {
+            "workload": f"config3: {len(wls3)} instances, {sum(w.asset.num_guide_strands for w in wls3)} strands "
+                        f"(log-uniform 500..50000 per instance; by vertices per strand {lengths}), {n3} vertices, "
+                        f"per-instance rigid trajectories and 64^3..128^3 SDFs rebuilt per frame from "
+                        f"5120-triangle noisy icospheres, 2 substeps; one graph replay per frame",
+            "value": bval, "unit": "vertex-substeps/s", "ms_per_step": bel * 1e3 / K3, "steps": K3,
+            "latency_ms": {"p50": float(np.percentile(bms, 50)), "p99": float(np.percentile(bms, 99))},
+            "plain": {"value": pval, "steps": P3} if pval else None,
+            "speedup_vs_plain": (bval / pval) if pval else None,
+        }
 
     cpu = None
     if not args.no_cpu:
-        nthreads = max(1, min(16, os.cpu_count() or 1))
         cwl = synth.config2(num_guides=args.guides)
-        cv, csub, cfr, cel = cpu_baseline(cwl, args.cpu_frames, nthreads, args.cpu_budget)
-        cpu = {"value": cv, "unit": "vertex-substeps/s", "cores": nthreads, "kind": "port",
+        cv, csub, cfr, cel = cpu_baseline(cwl, args.cpu_frames, host_workers, args.cpu_budget)
+        cpu = {"value": cv, "unit": "vertex-substeps/s", "cores": host_workers, "kind": "port",
                "sample": f"FP32 transcription on the host, {cfr} frames = {csub} substeps of the same workload "
                          f"(SDF build + simulate + collide), {cel:.1f} s wall"}
 
@@ -198,7 +273,8 @@ def main():
                         f"{wl.mesh.num_triangles}-triangle noisy icosphere, wind 3 m/s +-20% gust, "
                         f"{wl.substeps} substeps/frame, length iterations 2; step = one frame",
             "guide_strands": G, "follow_per_guide": F, "vertices_per_strand": V, "substeps": wl.substeps,
-            "path": "fused substep kernel (one persistent launch per substep)",
+            "path": "batched form, one member: per frame one hipGraph replay = parameter upload, "
+                    "SDF build (4 launches), 2 fused substep launches",
         },
         "latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99)),
                        "unit": "GPU ms per frame"},
@@ -215,11 +291,22 @@ def main():
             "launches": fused_n,
         },
         "sdf_build_ms_per_frame": sdf_ms / K,
+        "graph_equals_direct": graph_equals_direct,
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+    if args.copy_bytes > 0:
+        n = args.copy_bytes // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
 
 if __name__ == "__main__":

@@ -32,8 +32,10 @@ struct tfx_batch {
     tfx_simulation shape;              // launch-shape policy (variants per strand length)
 
     // Slot layout (host pinned slot and its device copy are identical):
-    //   DevParams[cap][n] | bones float[cap][bone_floats] | SdfJob[nsdf]
-    size_t off_bones = 0, off_jobs = 0, slot_bytes = 0;
+    //   SdfJob[nsdf] | substep 0 | substep 1 | ... | substep cap-1
+    // with substep s = DevParams[n] | bones float[bone_floats], so that one
+    // copy of the first off_sub + staged * sub_bytes bytes moves a frame.
+    size_t off_sub = 0, sub_params = 0, sub_bytes = 0, slot_bytes = 0;
     std::vector<int> bone_off;         // per member, floats into one substep's bone block
     int bone_floats = 0;
     std::vector<int> sdf_members;      // member indices that own an SDF
@@ -98,11 +100,11 @@ static int prepare(tfx_batch* b) {
             b->max_cells = std::max(b->max_cells, s->nx * s->ny * s->nz);
         }
     }
-    const size_t params_bytes = (size_t)b->cap * n * sizeof(DevParams);
-    b->off_bones = (params_bytes + 255) & ~size_t(255);
-    const size_t bones_bytes = (size_t)b->cap * b->bone_floats * sizeof(float);
-    b->off_jobs = (b->off_bones + bones_bytes + 255) & ~size_t(255);
-    b->slot_bytes = b->off_jobs + b->sdf_members.size() * sizeof(tfx::SdfJob);
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    b->off_sub = align(b->sdf_members.size() * sizeof(tfx::SdfJob));
+    b->sub_params = align((size_t)n * sizeof(DevParams));
+    b->sub_bytes = b->sub_params + align((size_t)b->bone_floats * sizeof(float));
+    b->slot_bytes = b->off_sub + (size_t)b->cap * b->sub_bytes;
     if (b->slot_bytes > b->host_bytes) {
         release_host(b);
         for (int k = 0; k < 2; ++k) {
@@ -117,9 +119,9 @@ static int prepare(tfx_batch* b) {
     for (int s = 0; s < b->cap; ++s)
         for (int m = 0; m < n; ++m) {
             DevHair d = dev_hair(b->members[m].hair, b->members[m].sdf);
-            d.params = reinterpret_cast<const DevParams*>(b->dev_slot) + (size_t)s * n + m;
-            d.bones = reinterpret_cast<const float*>((char*)b->dev_slot + b->off_bones) +
-                      (size_t)s * b->bone_floats + b->bone_off[m];
+            char* sub = (char*)b->dev_slot + b->off_sub + (size_t)s * b->sub_bytes;
+            d.params = reinterpret_cast<const DevParams*>(sub) + m;
+            d.bones = reinterpret_cast<const float*>(sub + b->sub_params) + b->bone_off[m];
             tables[(size_t)s * n + m] = d;
         }
     if (n > 0) {
@@ -228,7 +230,7 @@ extern "C" int tfx_batch_begin_frame(tfx_batch* b) {
     if (b->ev[k]) HIP_TRY(hipEventSynchronize(b->ev[k]));
     b->slot = k;
     b->staged = 0;
-    tfx::SdfJob* jobs = reinterpret_cast<tfx::SdfJob*>((char*)b->host[k] + b->off_jobs);
+    tfx::SdfJob* jobs = reinterpret_cast<tfx::SdfJob*>(b->host[k]);
     b->max_tris = 0;
     for (size_t j = 0; j < b->sdf_members.size(); ++j) {
         tfx_sdf* s = b->members[b->sdf_members[j]].sdf;
@@ -261,9 +263,9 @@ extern "C" int tfx_batch_stage_substep(tfx_batch* b, const float* bones, int bon
     const int n = (int)b->members.size();
     CHECK_ARG(n_settings == 1 || n_settings == n, "tfx_batch_stage_substep: settings count");
     CHECK_ARG(!bones || bones_stride >= 16, "tfx_batch_stage_substep: bones stride below one matrix");
-    char* host = (char*)b->host[b->slot];
-    DevParams* params = reinterpret_cast<DevParams*>(host) + (size_t)b->staged * n;
-    float* bone_block = reinterpret_cast<float*>(host + b->off_bones) + (size_t)b->staged * b->bone_floats;
+    char* sub = (char*)b->host[b->slot] + b->off_sub + (size_t)b->staged * b->sub_bytes;
+    DevParams* params = reinterpret_cast<DevParams*>(sub);
+    float* bone_block = reinterpret_cast<float*>(sub + b->sub_params);
     for (int m = 0; m < n; ++m) {
         tfx_hair* h = b->members[m].hair;
         if (bones) {
@@ -286,10 +288,11 @@ extern "C" int tfx_batch_stage_substep(tfx_batch* b, const float* bones, int bon
 
 static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
     const int n = (int)b->members.size();
-    HIP_TRY(hipMemcpyAsync(b->dev_slot, b->host[b->slot], b->slot_bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->dev_slot, b->host[b->slot], b->off_sub + (size_t)substeps * b->sub_bytes,
+                           hipMemcpyHostToDevice, s));
     const int nsdf = (int)b->sdf_members.size();
     if (nsdf) {
-        const tfx::SdfJob* jobs = reinterpret_cast<const tfx::SdfJob*>((char*)b->dev_slot + b->off_jobs);
+        const tfx::SdfJob* jobs = reinterpret_cast<const tfx::SdfJob*>(b->dev_slot);
         static const char* kNames[4] = {"InitializeSignedDistanceField", "ConstructSignedDistanceField",
                                         "FinalizeSignedDistanceField", "FinalizeSignedDistanceField"};
         for (int phase = 0; phase < 4; ++phase) {

@@ -95,3 +95,128 @@ class FrameDriver:
     def vertices(self):
         a = self.wl.asset
         return a.num_total_vertices
+
+
+class BatchDriver:
+    """Many workloads stepped together through the batched form (HairBatch):
+    per frame every collider's mesh is set, every member's substep inputs are
+    staged, and the frame is run -- directly or as a replayed hipGraph.
+    Per-frame host inputs (mesh descriptors, bone matrices, settings) are
+    built for the whole window of frames before any frame runs."""
+
+    def __init__(self, torch, workloads, frames, launch_shapes=None, meshes=None):
+        """meshes: ResidentMeshFrames per workload to share (same frames), or
+        None to upload them here."""
+        from . import HairBatch, SimulationSettings
+        self.wls = list(workloads)
+        self.frames = list(frames)
+        self.first = self.frames[0]
+        self.hairs, self.sdfs, self.meshes = [], [], []
+        self.batch = HairBatch()
+        for i, wl in enumerate(self.wls):
+            hair = TressFXHairObject(wl.asset)
+            if wl.capsules is not None:
+                hair.SetCollisionCapsules(*wl.capsules)
+            sdf = m = None
+            if wl.mesh is not None:
+                m = meshes[i] if meshes is not None else ResidentMeshFrames(torch, wl.mesh, wl.bone_fn, self.frames)
+                origin, cell, dims = wl.sdf_grid
+                sdf = TressFXSDFCollision(m.desc(0), collision_margin=wl.collision_margin,
+                                          explicit=(origin, cell, dims))
+            self.hairs.append(hair)
+            self.sdfs.append(sdf)
+            self.meshes.append(m)
+            self.batch.add(hair, sdf)
+        for V, variant in (launch_shapes or {}).items():
+            self.batch.set_launch_shape(V, variant)
+        subs = {wl.substeps for wl in self.wls}
+        assert len(subs) == 1, "members of one batch share the substep count"
+        self.substeps = subs.pop()
+        dt = 1.0 / synth.FRAME_RATE
+        self.dt_sub = np.float32(dt / self.substeps)
+        n = len(self.wls)
+        self.descs = [[m.desc(k) if m is not None else None for m in self.meshes] for k in range(len(self.frames))]
+        self.inputs = {}
+        for f in self.frames:
+            for s in range(self.substeps):
+                t = (f + (s + 1) / self.substeps) * dt
+                bones = np.stack([np.ascontiguousarray(wl.bone_fn(t), np.float32).reshape(16) for wl in self.wls])
+                st = (SimulationSettings * n)()
+                for m, wl in enumerate(self.wls):
+                    st[m] = wl.settings
+                    if wl.wind_fn is not None:
+                        st[m].wind_magnitude = wl.wind_fn(t)
+                self.inputs[(f, s)] = (bones, st)
+
+    @property
+    def vertices(self):
+        return sum(wl.asset.num_total_vertices for wl in self.wls)
+
+    def frame(self, f, stream=None, use_graph=False):
+        k = f - self.first
+        for sdf, d in zip(self.sdfs, self.descs[k]):
+            if sdf is not None:
+                sdf.SetMesh(d)
+        self.batch.begin_frame()
+        for s in range(self.substeps):
+            bones, st = self.inputs[(f, s)]
+            self.batch.stage_substep(bones, st, self.dt_sub)
+        self.batch.run_frame(stream, use_graph)
+
+
+class PlainMultiDriver:
+    """The same workloads stepped with the reference's per-object dispatch:
+    per frame each collider's SDF Update, then per substep each object's
+    inputs, Simulate over the objects and CollideWithHair per object."""
+
+    def __init__(self, torch, workloads, frames, meshes=None):
+        self.wls = list(workloads)
+        self.frames = list(frames)
+        self.first = self.frames[0]
+        self.sim = TressFXSimulation()
+        self.system = TressFXSDFCollisionSystem()
+        self.hairs, self.sdfs, self.meshes = [], [], []
+        for i, wl in enumerate(self.wls):
+            hair = TressFXHairObject(wl.asset)
+            if wl.capsules is not None:
+                hair.SetCollisionCapsules(*wl.capsules)
+            sdf = m = None
+            if wl.mesh is not None:
+                m = meshes[i] if meshes is not None else ResidentMeshFrames(torch, wl.mesh, wl.bone_fn, self.frames)
+                origin, cell, dims = wl.sdf_grid
+                sdf = TressFXSDFCollision(m.desc(0), collision_margin=wl.collision_margin,
+                                          explicit=(origin, cell, dims))
+            self.hairs.append(hair)
+            self.sdfs.append(sdf)
+            self.meshes.append(m)
+        self.substeps = self.wls[0].substeps
+        dt = 1.0 / synth.FRAME_RATE
+        self.dt_sub = np.float32(dt / self.substeps)
+        self.descs = [[m.desc(k) if m is not None else None for m in self.meshes] for k in range(len(self.frames))]
+        self.inputs = {}
+        for f in self.frames:
+            for s in range(self.substeps):
+                t = (f + (s + 1) / self.substeps) * dt
+                self.inputs[(f, s)] = [(np.ascontiguousarray(wl.bone_fn(t), np.float32).reshape(1, 16),
+                                        wl.wind_fn(t) if wl.wind_fn else None) for wl in self.wls]
+
+    @property
+    def vertices(self):
+        return sum(wl.asset.num_total_vertices for wl in self.wls)
+
+    def frame(self, f, stream=None):
+        k = f - self.first
+        for sdf, d in zip(self.sdfs, self.descs[k]):
+            if sdf is not None:
+                sdf.SetMesh(d)
+                sdf.Update(self.system, stream)
+        for s in range(self.substeps):
+            for hair, wl, (bones, wind) in zip(self.hairs, self.wls, self.inputs[(f, s)]):
+                hair.UpdateBoneMatrices(bones)
+                if wind is not None:
+                    wl.settings.wind_magnitude = wind
+                hair.UpdateSimulationParameters(wl.settings, self.dt_sub)
+            self.sim.Simulate(self.hairs, stream)
+            for hair, sdf in zip(self.hairs, self.sdfs):
+                if sdf is not None:
+                    sdf.CollideWithHair(self.system, hair, stream)

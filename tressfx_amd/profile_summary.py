@@ -1,0 +1,87 @@
+"""Summaries of rocprofv3 CSV output for one kernel: duration statistics from
+a --kernel-trace --stats run and HBM bytes per dispatch from --pmc runs of
+FETCH_SIZE and WRITE_SIZE, calibrated against a device copy of known size.
+
+    python -m tressfx_amd.profile_summary stats  KERNEL_STATS_CSV KERNEL_SUBSTRING
+    python -m tressfx_amd.profile_summary traffic FETCH_CSV WRITE_CSV KERNEL_SUBSTRING COPY_BYTES OUT_JSON
+
+Counter values are KiB per dispatch.  The copy kernel of the same run
+(__amd_rocclr_copyBuffer, COPY_BYTES read and written) gives the scale factor
+counter_bytes / true_bytes for each counter; a kernel's traffic is its counter
+bytes divided by that factor.
+"""
+import csv
+import json
+import statistics
+import sys
+
+
+def _rows(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def kernel_stats(path, kernel):
+    out = []
+    for r in _rows(path):
+        if kernel in r["Name"]:
+            out.append({"name": r["Name"], "calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3,
+                        "min_us": float(r["MinNs"]) / 1e3, "max_us": float(r["MaxNs"]) / 1e3,
+                        "total_ms": float(r["TotalDurationNs"]) / 1e6})
+    return out
+
+
+def _per_dispatch(path, counter):
+    """{kernel name: [counter value per dispatch]} (values summed over the
+    counter's instances within a dispatch)."""
+    acc = {}
+    for r in _rows(path):
+        if r.get("Counter_Name") != counter:
+            continue
+        key = (r["Kernel_Name"], r["Dispatch_Id"])
+        acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
+    per = {}
+    for (name, _), v in acc.items():
+        per.setdefault(name, []).append(v)
+    return per
+
+
+def traffic(fetch_csv, write_csv, kernel, copy_bytes):
+    res = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        per = _per_dispatch(path, counter)
+        copies = [v for n, vs in per.items() if "copyBuffer" in n for v in vs]
+        # the calibration copies are the largest copyBuffer dispatches
+        copies = sorted(copies)[-3:]
+        if not copies:
+            raise SystemExit(f"{path}: no copy kernel dispatches to calibrate {counter}")
+        factor = statistics.mean(copies) * 1024.0 / copy_bytes
+        ks = [v for n, vs in per.items() if kernel in n for v in vs]
+        if not ks:
+            raise SystemExit(f"{path}: no dispatches of {kernel}")
+        res[counter] = {"kib_per_dispatch_mean": statistics.mean(ks), "dispatches": len(ks),
+                        "copy_kib": statistics.mean(copies), "factor": factor,
+                        "bytes_per_dispatch": statistics.mean(ks) * 1024.0 / factor}
+    res["bytes_per_launch"] = res["FETCH_SIZE"]["bytes_per_dispatch"] + res["WRITE_SIZE"]["bytes_per_dispatch"]
+    res["kernel"] = kernel
+    res["copy_bytes"] = copy_bytes
+    return res
+
+
+def main(argv):
+    if argv[:1] == ["stats"]:
+        for k in kernel_stats(argv[1], argv[2]):
+            print(json.dumps(k))
+    elif argv[:1] == ["traffic"]:
+        r = traffic(argv[1], argv[2], argv[3], int(argv[4]))
+        with open(argv[5], "w") as f:
+            json.dump(r, f, indent=1)
+        print(json.dumps(r))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

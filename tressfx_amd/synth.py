@@ -73,10 +73,7 @@ def strand_tangents(p32):
 
     def nz(v):
         d = np.sqrt((v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]) + v[..., 2] * v[..., 2])
-        out = v.copy()
-        ok = d != 0
-        out[ok] = v[ok] / d[ok][:, None]
-        return out
+        return v / np.where(d != 0, d, np.float32(1.0))[..., None]
 
     t = np.zeros_like(p32)
     t[:, 0] = nz(p32[:, 1] - p32[:, 0])
@@ -224,7 +221,7 @@ class NoisySphereMesh:
     def num_triangles(self):
         return self.indices.shape[0]
 
-    def vertices(self, t, bone=None):
+    def _displaced(self, t):
         ph = 2.0 * math.pi * self.anim_hz * t
         shift = 0.5 * np.array([math.cos(ph), math.sin(ph), 0.3 * math.cos(ph)])
         n = self.unit
@@ -235,7 +232,17 @@ class NoisySphereMesh:
             disp += a * self.noise(n * self.freq * (2 ** o) + shift * (2 ** o) + 17.0 * o)
             norm += a
         r = self.radius + self.amp * disp / norm
-        p = n * self.scale * r[:, None]
+        return n * self.scale * r[:, None]
+
+    def vertices(self, t, bone=None):
+        if self.anim_hz == 0.0:
+            # static displacement: evaluated once, then only moved rigidly
+            if getattr(self, "_static", None) is None:
+                self._static = self._displaced(0.0)
+            p = self._static
+        else:
+            p = self._displaced(t)
+        n = self.unit
         if bone is not None:
             ph = np.concatenate([p, np.ones((p.shape[0], 1))], axis=1) @ bone.astype(np.float64)
             p = ph[:, :3]
@@ -312,3 +319,78 @@ def config2(num_guides=20000, V=32, F=4, grid_n=128, cell=0.0025, subdivisions=5
     half = 0.5 * grid_n * cell
     grid = ((-half, -half, -half), cell, (grid_n, grid_n, grid_n))
     return Workload(asset, settings_config2(), 2, head_yaw, mesh=mesh, sdf_grid=grid, wind_fn=wind_gust)
+
+
+def rigid_trajectory(rng):
+    """Independent yaw and pitch sinusoids: amplitude U(10, 45) degrees,
+    frequency U(0.3, 2) Hz, random phase."""
+    amp = np.radians(rng.uniform(10.0, 45.0, 2))
+    hz = rng.uniform(0.3, 2.0, 2)
+    ph = rng.uniform(0.0, 2.0 * math.pi, 2)
+
+    def bone(t):
+        return yaw_pitch_matrix(amp[0] * math.sin(2.0 * math.pi * hz[0] * t + ph[0]),
+                                amp[1] * math.sin(2.0 * math.pi * hz[1] * t + ph[1]))
+    return bone
+
+
+CONFIG3_LENGTHS = (16, 32, 64)
+CONFIG3_LENGTH_P = (0.5, 0.35, 0.15)
+CONFIG3_GRIDS = (64, 80, 96, 112, 128)
+CONFIG3_EXTENT = 0.32     # SDF grid edge in metres (128 cells at 2.5 mm)
+
+
+def config3_spec(num_instances=256, seed=0, min_strands=500, max_strands=50000):
+    """Per-instance (guide strands, vertices per strand, SDF cells per edge):
+    strands log-uniform in [min_strands, max_strands], lengths from
+    {16, 32, 64} with p = {0.5, 0.35, 0.15}, SDF 64^3..128^3."""
+    rng = rng_for(3, 100000 + seed)
+    G = np.exp(rng.uniform(math.log(min_strands), math.log(max_strands), num_instances)).round().astype(int)
+    V = rng.choice(CONFIG3_LENGTHS, size=num_instances, p=CONFIG3_LENGTH_P)
+    n = rng.choice(CONFIG3_GRIDS, size=num_instances)
+    return [(int(g), int(v), int(c)) for g, v, c in zip(G, V, n)]
+
+
+def _c3_rng(i, seed, stream):
+    return np.random.Generator(np.random.PCG64([3, seed, i, stream]))
+
+
+def config3_asset(i, G, V, seed=0):
+    """Strands of instance i of the batched workload (no follow strands)."""
+    rng = _c3_rng(i, seed, 0)
+    roots, nrm = hemisphere_roots(G, 0.1, rng)
+    strands = grow_strands(roots, nrm, V, 0.005, 0.15, rng)
+    return build_asset(strands, 0, rng)
+
+
+def config3_instance(i, G, V, grid_n, seed=0, subdivisions=4, asset=None):
+    """One instance of the batched workload: its own strands, rigid head
+    trajectory and noisy-icosphere collider (static noise, moved rigidly);
+    config-2 solver settings and wind, 2 substeps, no follow strands."""
+    if asset is None:
+        asset = config3_asset(i, G, V, seed)
+    rng = _c3_rng(i, seed, 1)
+    mesh = NoisySphereMesh(rng, radius=0.1, subdivisions=subdivisions, amp=0.01, anim_hz=0.0)
+    cell = CONFIG3_EXTENT / grid_n
+    half = 0.5 * CONFIG3_EXTENT
+    grid = ((-half, -half, -half), cell, (grid_n, grid_n, grid_n))
+    return Workload(asset, settings_config2(), 2, rigid_trajectory(rng), mesh=mesh, sdf_grid=grid,
+                    wind_fn=wind_gust)
+
+
+def _c3_asset_job(args):
+    return config3_asset(*args)
+
+
+def config3(num_instances=256, seed=0, workers=1, **kw):
+    """The batched workload; assets are generated by `workers` forked
+    processes (call before the process touches the GPU)."""
+    spec = config3_spec(num_instances, seed, **kw)
+    jobs = [(i, G, V, seed) for i, (G, V, n) in enumerate(spec)]
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            assets = pool.map(_c3_asset_job, jobs, chunksize=1)
+    else:
+        assets = [config3_asset(*j) for j in jobs]
+    return [config3_instance(i, G, V, n, seed, asset=a) for (i, (G, V, n)), a in zip(enumerate(spec), assets)]
