@@ -1,0 +1,216 @@
+// shim_test.cpp — drives the C++ classes of tressfx_amd.hpp through the
+// reference's per-frame order (src/Simulation.cpp:50-131: SDF Update, then
+// Simulate and CollideWithHair) and, side by side, the same frames through the
+// plain C-ABI calls on a second set of objects; the two must end bit-identical.
+// Also checks the getters and that a bad asset raises tressfx_amd::Error.
+// Prints "shim_test OK" and exits 0 on success.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "tressfx_amd.hpp"
+
+using namespace tressfx_amd;
+
+namespace {
+
+#define REQUIRE(c)                                                            \
+    do {                                                                      \
+        if (!(c)) {                                                           \
+            std::fprintf(stderr, "shim_test: %s failed at line %d\n", #c, __LINE__); \
+            return 1;                                                         \
+        }                                                                     \
+    } while (0)
+
+// Guide strands straight out of a sphere of radius 0.1 along the surface
+// normal (a Fibonacci lattice on the upper hemisphere), one follow strand each.
+TressFXAsset make_asset(int G, int V, int F) {
+    TressFXAsset a;
+    a.m_numGuideStrands = G;
+    a.m_numFollowStrandsPerGuide = F;
+    a.m_numVerticesPerStrand = V;
+    const int S = G * (F + 1);
+    a.m_positions.assign((size_t)S * V * 4, 0.f);
+    a.m_tangents.assign((size_t)S * V * 4, 0.f);
+    a.m_restLengths.assign((size_t)S * V, 0.f);
+    a.m_followRootOffsets.assign((size_t)S * 4, 0.f);
+    a.m_boneSkinningData.assign(S, tfx_bone_skinning_data{{0, 0, 0, 0}, {1, 0, 0, 0}});
+    const float seg = 0.005f;
+    for (int g = 0; g < G; ++g) {
+        const float y = (g + 0.5f) / G;
+        const float r = std::sqrt(1.f - y * y), phi = 2.39996323f * g;
+        const float n[3] = {r * std::cos(phi), y, r * std::sin(phi)};
+        for (int j = 0; j <= F; ++j) {
+            const int s = g * (F + 1) + j;
+            const float off[3] = {j ? 0.002f * n[2] : 0.f, 0.f, j ? -0.002f * n[0] : 0.f};
+            a.m_followRootOffsets[4 * s + 0] = off[0];
+            a.m_followRootOffsets[4 * s + 2] = off[2];
+            a.m_followRootOffsets[4 * s + 3] = (float)(g * (F + 1));
+            for (int v = 0; v < V; ++v) {
+                float* p = &a.m_positions[((size_t)s * V + v) * 4];
+                for (int k = 0; k < 3; ++k) p[k] = n[k] * (0.1f + seg * v) + off[k];
+                p[3] = v < 2 ? 0.f : 1.f;
+                float* t = &a.m_tangents[((size_t)s * V + v) * 4];
+                for (int k = 0; k < 3; ++k) t[k] = v < V - 1 ? n[k] : 0.f;
+                a.m_restLengths[(size_t)s * V + v] = v < V - 1 ? seg : 0.f;
+            }
+        }
+    }
+    return a;
+}
+
+// An octahedron of radius 0.1 at the origin, moved by a translation per frame.
+class OctaMesh : public TressFXSDFInputMeshInterface {
+public:
+    OctaMesh() {
+        (void)hipMalloc(&verts_, 6 * sizeof(tfx_standard_vertex));
+        (void)hipMalloc(&idx_, 24 * sizeof(unsigned));
+        // outward counter-clockwise winding
+        const unsigned idx[24] = {0, 4, 2, 2, 4, 1, 1, 4, 3, 3, 4, 0, 0, 2, 5, 2, 1, 5, 1, 3, 5, 3, 0, 5};
+        (void)hipMemcpy(idx_, idx, sizeof(idx), hipMemcpyHostToDevice);
+        set(0.f);
+        for (int a = 0; a < 3; ++a) { imin_[a] = min_[a]; imax_[a] = max_[a]; }
+    }
+    ~OctaMesh() override {
+        (void)hipFree(verts_);
+        (void)hipFree(idx_);
+    }
+    void set(float dx) {
+        const float r = 0.1f;
+        const float p[6][3] = {{r, 0, 0}, {-r, 0, 0}, {0, 0, r}, {0, 0, -r}, {0, r, 0}, {0, -r, 0}};
+        tfx_standard_vertex v[6];
+        std::memset(v, 0, sizeof(v));
+        for (int i = 0; i < 6; ++i) {
+            v[i].position[0] = p[i][0] + dx;
+            v[i].position[1] = p[i][1];
+            v[i].position[2] = p[i][2];
+            v[i].position[3] = 1.f;
+        }
+        (void)hipMemcpy(verts_, v, sizeof(v), hipMemcpyHostToDevice);
+        min_[0] = -r + dx; max_[0] = r + dx;
+        min_[1] = min_[2] = -r;
+        max_[1] = max_[2] = r;
+    }
+    const tfx_standard_vertex* GetMeshBuffer() override { return (const tfx_standard_vertex*)verts_; }
+    const unsigned* GetTrimeshVertexIndicesBuffer() override { return (const unsigned*)idx_; }
+    int GetNumMeshVertices() override { return 6; }
+    int GetNumMeshTriangle() override { return 8; }
+    void GetBoundingBox(float mn[3], float mx[3]) override {
+        for (int a = 0; a < 3; ++a) { mn[a] = min_[a]; mx[a] = max_[a]; }
+    }
+    void GetInitialBoundingBox(float mn[3], float mx[3]) override {
+        for (int a = 0; a < 3; ++a) { mn[a] = imin_[a]; mx[a] = imax_[a]; }
+    }
+
+private:
+    void* verts_ = nullptr;
+    void* idx_ = nullptr;
+    float min_[3], max_[3], imin_[3], imax_[3];
+};
+
+void yaw_matrix(float yaw, float m[16]) {
+    // row-vector convention: p' = [p,1] * M, translation in row 3
+    std::memset(m, 0, 16 * sizeof(float));
+    const float c = std::cos(yaw), s = std::sin(yaw);
+    m[0] = c;  m[2] = -s;
+    m[5] = 1.f;
+    m[8] = s;  m[10] = c;
+    m[15] = 1.f;
+}
+
+}  // namespace
+
+int main() {
+    try {
+        TressFXSimulation::Initialize(0);
+        const int G = 200, V = 16, F = 1;
+        TressFXAsset asset = make_asset(G, V, F);
+
+        // The C++ classes.
+        TressFXHairObject hairA(&asset);
+        REQUIRE(hairA.GetNumTotalHairStrands() == G * (F + 1));
+        REQUIRE(hairA.GetNumTotalHairVertices() == G * (F + 1) * V);
+        REQUIRE(hairA.GetNumVerticesPerStrand() == V);
+        REQUIRE(hairA.GetNumFollowHairsPerGuideHair() == F);
+        OctaMesh mesh;
+        TressFXSDFCollisionSystem system;
+        TressFXSDFCollision sdfA(&mesh, 48, 1.0f);
+        int nx, ny, nz;
+        sdfA.GetGridNumCells(nx, ny, nz);
+        REQUIRE(nx > 0 && ny > 0 && nz > 0 && sdfA.GetGridNumTotalCells() == nx * ny * nz);
+        TressFXSimulation sim;
+
+        // The same frames through the C-ABI directly.
+        const tfx_asset_desc d = asset.desc();
+        tfx_hair* hairB = nullptr;
+        check(tfx_hair_create(&d, &hairB));
+        const tfx_sdf_mesh_desc md = mesh.desc();
+        tfx_sdf* sdfB = nullptr;
+        check(tfx_sdf_create(&md, 48, 1.0f, &sdfB));
+        tfx_simulation* simB = nullptr;
+        check(tfx_simulation_create(&simB));
+        tfx_sdf_collision_system* sysB = nullptr;
+        check(tfx_sdf_collision_system_create(&sysB));
+
+        TressFXSimulationSettings settings;
+        settings.gravity_magnitude = 9.8f;
+        settings.wind_magnitude = 2.0f;
+        std::vector<TressFXHairObject*> objs{&hairA};
+        const float dt = 1.f / 60.f;
+        for (int frame = 0; frame < 20; ++frame) {
+            float bone[16];
+            yaw_matrix(0.5f * std::sin(0.3f * frame), bone);
+            mesh.set(0.002f * frame);
+            sdfA.Update(nullptr, system);
+            hairA.UpdateBoneMatrices(bone, 1);
+            hairA.UpdateSimulationParameters(&settings, dt);
+            sim.Simulate(nullptr, objs);
+            sdfA.CollideWithHair(nullptr, system, hairA);
+
+            const tfx_sdf_mesh_desc m2 = mesh.desc();
+            check(tfx_sdf_set_mesh(sdfB, &m2));
+            check(tfx_sdf_update(sdfB, sysB, nullptr));
+            check(tfx_hair_update_bone_matrices(hairB, bone, 1));
+            check(tfx_hair_update_simulation_parameters(hairB, &settings, dt));
+            check(tfx_simulation_simulate(simB, nullptr, &hairB, 1));
+            check(tfx_sdf_collide_with_hair(sdfB, sysB, nullptr, hairB));
+        }
+        const size_t n4 = (size_t)G * (F + 1) * V * 4;
+        std::vector<float> pa(n4), pb(n4), qa(n4), qb(n4);
+        tfx_dynamic_state st = hairA.GetDynamicState();
+        REQUIRE(st.positions != nullptr);
+        check(tfx_hair_download_state(hairA.handle(), nullptr, pa.data(), qa.data(), nullptr, nullptr, nullptr));
+        check(tfx_hair_download_state(hairB, nullptr, pb.data(), qb.data(), nullptr, nullptr, nullptr));
+        REQUIRE(std::memcmp(pa.data(), pb.data(), n4 * sizeof(float)) == 0);
+        REQUIRE(std::memcmp(qa.data(), qb.data(), n4 * sizeof(float)) == 0);
+        double moved = 0;
+        for (size_t i = 0; i < n4; ++i) {
+            REQUIRE(std::isfinite(pa[i]));
+            moved = std::fmax(moved, std::fabs((double)pa[i] - asset.m_positions[i]));
+        }
+        REQUIRE(moved > 1e-3);
+
+        tfx_sdf_collision_system_destroy(sysB);
+        tfx_simulation_destroy(simB);
+        tfx_sdf_destroy(sdfB);
+        tfx_hair_destroy(hairB);
+
+        // A bad asset is an error, not an abort.
+        TressFXAsset bad = make_asset(4, 12, 0);
+        bool threw = false;
+        try {
+            TressFXHairObject h(&bad);
+        } catch (const Error& e) {
+            threw = e.code() == TFX_ERR_INVALID_ARGUMENT;
+        }
+        REQUIRE(threw);
+    } catch (const Error& e) {
+        std::fprintf(stderr, "shim_test: %s (code %d)\n", e.what(), e.code());
+        return 1;
+    }
+    std::printf("shim_test OK\n");
+    return 0;
+}

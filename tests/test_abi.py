@@ -126,3 +126,14 @@ def test_asset_validation_rejects_bad_shapes():
     d = a.desc()
     assert lib.tfx_hair_create(ctypes.byref(d), ctypes.byref(h)) == 1
     assert b"bone index" in lib.tfx_last_error()
+
+
+def test_cpp_shim_compiles_against_the_header(tmp_path):
+    """include/tressfx_amd.hpp (the reference-named C++ classes) and the shim
+    test program compile warning-free and link against the library."""
+    import subprocess
+    from tressfx_amd import build
+    out = str(tmp_path / "shim_test")
+    res = subprocess.run(build.cpp_test_command(out), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert res.returncode == 0, res.stdout
+    assert os.path.exists(out)

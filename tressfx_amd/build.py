@@ -37,6 +37,33 @@ FLAGS = [
 ]
 
 
+CPP_TEST_SRC = os.path.join(REPO_DIR, "tests", "cpp", "shim_test.cpp")
+CPP_TEST_BIN = os.path.join(REPO_DIR, "tests", "cpp", "shim_test")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def cpp_test_command(out=CPP_TEST_BIN):
+    """g++ line of the C++ shim test program: the C++ classes of
+    include/tressfx_amd.hpp over libtressfx_amd.so (found next to the package
+    through a relative rpath)."""
+    rpath = os.path.relpath(PKG_DIR, os.path.dirname(os.path.abspath(out)))
+    return ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
+            "-I", os.path.join(REPO_DIR, "include"), "-I", os.path.join(ROCM, "include"), CPP_TEST_SRC,
+            "-L", PKG_DIR, "-ltressfx_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+            "-Wl,-rpath,$ORIGIN/" + rpath, "-o", out]
+
+
+def build_cpp_test(verbose=False, out=CPP_TEST_BIN):
+    cmd = cpp_test_command(out)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError("g++ failed building the C++ shim test")
+    return out
+
+
 def build_library(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB_PATH + ".tmp"
@@ -53,3 +80,4 @@ def build_library(verbose=False):
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_cpp_test(verbose=True))
