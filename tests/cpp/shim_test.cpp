@@ -140,7 +140,8 @@ int main() {
         TressFXSDFCollision sdfA(&mesh, 48, 1.0f);
         int nx, ny, nz;
         sdfA.GetGridNumCells(nx, ny, nz);
-        REQUIRE(nx > 0 && ny > 0 && nz > 0 && sdfA.GetGridNumTotalCells() == nx * ny * nz);
+        // m_NumTotalCells is the allocation: 1.4 x the grid (TressFXSDFCollision.cpp:68-70)
+        REQUIRE(nx > 0 && ny > 0 && nz > 0 && sdfA.GetGridNumTotalCells() >= nx * ny * nz);
         TressFXSimulation sim;
 
         // The same frames through the C-ABI directly.
