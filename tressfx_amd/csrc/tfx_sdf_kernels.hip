@@ -9,7 +9,8 @@
 //
 // Construct maps a team of lanes to one triangle and spreads the triangle's
 // cell box over the team (instead of one thread looping over the whole box), so
-// large and small triangles share a wave without one lane serialising it.
+// large and small triangles share a wave without one lane serialising it; a
+// node's atomic is skipped when its current value is already no larger.
 #include "tfx_device.h"
 
 namespace tfx {
@@ -118,15 +119,15 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_bricks(const uint8_t* __
     bits[w] = m;
 }
 
-// ConstructSignedDistanceField :329-397.  kTeam lanes per triangle; the team
-// walks the triangle's node box in x-fastest order.
+// ConstructSignedDistanceField :329-397 for one triangle, spread over the
+// kTeam lanes of a team: the triangle's padded node box (AABB +- one cell,
+// GetSdfCoordinates, -1/+1, clamped) walked in x-fastest order.  A node's
+// current value is read first (agent-coherent load) and the atomicMin issued
+// only if the new value is smaller: the field only decreases, so a value read
+// at any time bounds the final one from above and skipping is exact.
 template <int kTeam>
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_construct(uint32_t* __restrict__ sdf,
-                                                             const tfx_standard_vertex* __restrict__ verts,
-                                                             const uint32_t* __restrict__ idx, int ntri, SdfGrid g) {
-    const int lane = threadIdx.x % kTeam;
-    const int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam;
-    if (tri >= ntri) return;
+__device__ __forceinline__ void construct_triangle(uint32_t* __restrict__ cells, const tfx_standard_vertex* verts,
+                                                   const uint32_t* idx, int tri, const SdfGrid& g, int lane) {
     const uint32_t i0 = idx[3 * tri + 0], i1 = idx[3 * tri + 1], i2 = idx[3 * tri + 2];
     const f3 x0{verts[i0].position[0], verts[i0].position[1], verts[i0].position[2]};
     const f3 x1{verts[i1].position[0], verts[i1].position[1], verts[i1].position[2]};
@@ -148,16 +149,40 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct(uint32_t* __restric
     const TriPrep tp = tri_prep(x0, x1, x2);
     const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
     const int total = ex * ey * ez;
+    // Exact small-integer division through the float reciprocal: for
+    // 0 <= c < 2^16, (c + 0.5) / e stays more than 0.5/e - 2^-8/e from any
+    // integer, beyond the rounding of the two float operations.
+    const bool fast = total < 65536;
+    const float inv_x = 1.0f / (float)ex, inv_y = 1.0f / (float)ey;
     for (int c = lane; c < total; c += kTeam) {
-        const int xx = c % ex;
-        const int rem = c / ex;
-        const int yy = rem % ey;
-        const int zz = rem / ey;
+        int rem, xx, yy, zz;
+        if (fast) {
+            rem = (int)(((float)c + 0.5f) * inv_x);
+            xx = c - rem * ex;
+            zz = (int)(((float)rem + 0.5f) * inv_y);
+            yy = rem - zz * ey;
+        } else {
+            xx = c % ex;
+            rem = c / ex;
+            yy = rem % ey;
+            zz = rem / ey;
+        }
         const int x = lo[0] + xx, y = lo[1] + yy, z = lo[2] + zz;
-        const f3 cp = sdf_cell_pos(g, x, y, z);
-        const float d = signed_distance(tp, cp);
-        atomicMin(&sdf[sdf_index(g, x, y, z)], float_flip3(d));
+        const float d = signed_distance(tp, sdf_cell_pos(g, x, y, z));
+        const uint32_t f = float_flip3(d);
+        uint32_t* cell = &cells[sdf_index(g, x, y, z)];
+        if (f < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(cell, f);
     }
+}
+
+// ConstructSignedDistanceField :329-397: kTeam lanes per triangle.
+template <int kTeam>
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_construct(uint32_t* __restrict__ sdf,
+                                                             const tfx_standard_vertex* __restrict__ verts,
+                                                             const uint32_t* __restrict__ idx, int ntri, SdfGrid g) {
+    const int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam;
+    if (tri >= ntri) return;
+    construct_triangle<kTeam>(sdf, verts, idx, tri, g, threadIdx.x % kTeam);
 }
 
 // CollideHairVerticesWithSdf_forward :531-595 — one lane per vertex, guide
@@ -241,38 +266,8 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob*
     const SdfJob& j = jobs[blockIdx.y];
     const SdfGrid g = j.g;
     const int lane = threadIdx.x % kTeam;
-    for (int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam; tri < j.ntri; tri += gridDim.x * (kSdfBlock / kTeam)) {
-        const uint32_t i0 = j.idx[3 * tri + 0], i1 = j.idx[3 * tri + 1], i2 = j.idx[3 * tri + 2];
-        const f3 x0{j.verts[i0].position[0], j.verts[i0].position[1], j.verts[i0].position[2]};
-        const f3 x1{j.verts[i1].position[0], j.verts[i1].position[1], j.verts[i1].position[2]};
-        const f3 x2{j.verts[i2].position[0], j.verts[i2].position[1], j.verts[i2].position[2]};
-        const float m = g.cell;
-        f3 mn{fmn(x0.x, fmn(x1.x, x2.x)) - m, fmn(x0.y, fmn(x1.y, x2.y)) - m, fmn(x0.z, fmn(x1.z, x2.z)) - m};
-        f3 mx{fmx(x0.x, fmx(x1.x, x2.x)) + m, fmx(x0.y, fmx(x1.y, x2.y)) + m, fmx(x0.z, fmx(x1.z, x2.z)) + m};
-        int lo[3], hi[3];
-        sdf_coords(g, mn, lo[0], lo[1], lo[2]);
-        sdf_coords(g, mx, hi[0], hi[1], hi[2]);
-        const int dims[3] = {g.nx, g.ny, g.nz};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = (int)((uint32_t)lo[k] - 1u);
-            hi[k] = (int)((uint32_t)hi[k] + 1u);
-            lo[k] = max(0, min(lo[k], dims[k] - 1));
-            hi[k] = max(0, min(hi[k], dims[k] - 1));
-        }
-        const TriPrep tp = tri_prep(x0, x1, x2);
-        const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
-        const int total = ex * ey * ez;
-        for (int c = lane; c < total; c += kTeam) {
-            const int xx = c % ex;
-            const int rem = c / ex;
-            const int yy = rem % ey;
-            const int zz = rem / ey;
-            const int x = lo[0] + xx, y = lo[1] + yy, z = lo[2] + zz;
-            const float d = signed_distance(tp, sdf_cell_pos(g, x, y, z));
-            atomicMin(&j.cells[sdf_index(g, x, y, z)], float_flip3(d));
-        }
-    }
+    for (int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam; tri < j.ntri; tri += gridDim.x * (kSdfBlock / kTeam))
+        construct_triangle<kTeam>(j.cells, j.verts, j.idx, tri, g, lane);
 }
 
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* __restrict__ jobs) {
