@@ -306,6 +306,36 @@ int tfx_sdf_download(tfx_sdf* sdf, void* stream, float* host_cells /* X*Y*Z */);
 int tfx_sdf_get_collision_params(const tfx_sdf* sdf, const tfx_hair* hair, tfx_sdf_collision_params* out);
 
 /* ------------------------------------------------------------------------ */
+/* Collision-mesh skinning (TressFXBoneSkinning, src/TressFX/TressFXBoneSkinning.h:50-131) */
+/* ------------------------------------------------------------------------ */
+typedef struct tfx_mesh_skinning tfx_mesh_skinning;
+/* The content of a loaded .tfxmesh (LoadTressFXCollisionMeshData,
+ * TressFXBoneSkinning.cpp:100-286) with bone indices already mapped to engine
+ * bone ids (GetBoneIdByName).  Arrays are copied. */
+typedef struct tfx_skinned_mesh_desc {
+    int num_vertices;
+    int num_triangles;
+    const float* positions;                /* float3[num_vertices] rest positions */
+    const float* normals;                  /* float3[num_vertices] or NULL */
+    const tfx_bone_skinning_data* skinning; /* [num_vertices] engine bone ids (as float), weights */
+    const uint32_t* indices;               /* [3 * num_triangles] */
+    int follow_bone;                       /* engine bone the bounding box follows (m_followBone) */
+    int pad;
+} tfx_skinned_mesh_desc;
+/* Initialize, TressFXBoneSkinning.cpp:287-399 (device buffers, bounding cube). */
+int tfx_mesh_skinning_create(const tfx_skinned_mesh_desc* mesh, tfx_mesh_skinning** out);
+int tfx_mesh_skinning_destroy(tfx_mesh_skinning* mesh);
+/* Update, TressFXBoneSkinning.cpp:460-504: new world-space bone matrices
+ * (row-major row-vector float4x4), then BoneSkinning over every vertex. */
+int tfx_mesh_skinning_update(tfx_mesh_skinning* mesh, const float* bone_matrices, int num_bones, void* stream);
+/* The TressFXSDFInputMeshInterface view of the skinned mesh (.h:70-81):
+ * device buffers, GetBoundingBox (:439-452) and GetInitialBoundingBox (:454-458),
+ * for tfx_sdf_set_mesh. */
+int tfx_mesh_skinning_get_mesh(const tfx_mesh_skinning* mesh, tfx_sdf_mesh_desc* out);
+/* Copies the skinned vertex buffer (StandardVertex[num_vertices]) to the host. */
+int tfx_mesh_skinning_download(tfx_mesh_skinning* mesh, void* stream, tfx_standard_vertex* host_vertices);
+
+/* ------------------------------------------------------------------------ */
 /* Batched form (no counterpart in the reference, whose Simulate dispatches  */
 /* every stage once per object, TressFXSimulation.cpp:51-75): many objects   */
 /* of uneven size advance in one persistent launch per substep and strand    */

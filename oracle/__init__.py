@@ -48,6 +48,8 @@ def load():
     for p in ("32", "64"):
         getattr(lib, f"orc{p}_sdf_build").argtypes = [_VP, _VP, _I, _VP, R[p], _I, _I, _I, _VP, _I]
         getattr(lib, f"orc{p}_sdf_collide").argtypes = [_VP, _VP, _I, _I, _VP, _VP, R[p], _I, _I, _I, R[p], _I]
+    for p in ("32", "64"):
+        getattr(lib, f"orc{p}_mesh_skin").argtypes = [_VP, _VP, _I, _VP, _VP, _I]
     lib.orc32_get_signed_distance.argtypes = [_VP, _VP, _F, _I, _I, _I, _VP]
     lib.orc32_get_signed_distance.restype = _F
     lib.orc_float_flip3.argtypes = [_F]
@@ -145,6 +147,23 @@ def sdf_collide(pos, prev, V, sdf, origin, cell, dims, margin, nthreads=0):
     n = pos.shape[0]
     fn(_p(pos), _p(prev), n, V, _p(np.ascontiguousarray(sdf, dt)), _p(o), cell, int(dims[0]), int(dims[1]),
        int(dims[2]), margin, nthreads)
+
+
+def mesh_skin(rest, skin, bones, dtype=np.float32, nthreads=0):
+    """BoneSkinning transcription: rest StandardVertex float32[n, 8], skin
+    float32[n, 8] (bone ids as float, weights), bones float32[512, 16] ->
+    skinned float[n, 8] of `dtype`."""
+    lib = load()
+    r = np.ascontiguousarray(rest, np.float32).reshape(-1, 8)
+    sk = np.ascontiguousarray(skin, np.float32).reshape(-1, 8)
+    b = np.zeros((512, 16), np.float32)
+    bb = np.asarray(bones, np.float32).reshape(-1, 16)
+    b[:bb.shape[0]] = bb
+    b[bb.shape[0]:] = np.eye(4, dtype=np.float32).reshape(16)
+    out = np.zeros(r.shape, np.dtype(dtype))
+    fn = lib.orc64_mesh_skin if np.dtype(dtype) == np.float64 else lib.orc32_mesh_skin
+    fn(_p(r), _p(sk), r.shape[0], _p(b), _p(out), nthreads)
+    return out
 
 
 def pack_sim_params(settings, dt, frame, V, F, into=None):

@@ -23,7 +23,8 @@
 // global shape, VSP, LSC fixed points, length projection, wind, follow hair,
 // FloatFlip3 ordering, point–triangle distance (Ericson's closest point in
 // FP64), SDF order independence and the plane-SDF collision response.
-// Host parameter packing (SetWind) is pinned against closed-form rotations.
+// Host parameter packing (SetWind) is pinned against closed-form rotations;
+// collision-mesh skinning against rigid transforms and matrix blends.
 
 #include <algorithm>
 #include <atomic>
@@ -786,6 +787,24 @@ template <class R> static Hair<R> to_hair(const orc_hair* o) {
     return h;
 }
 
+// BoneSkinning, TressFXBoneSkinning.hlsl:57-94: every collision-mesh vertex
+// (StandardVertex: float4 position, float4 normal) by the weight-normalised
+// bone blend; position [p,1]·M and normal [n,0]·M, w components kept.
+template <class R> void mesh_skin(const float* rest, const float* skin, int n, const float* bones, R* out,
+                                  int nthreads) {
+    parallel_for(n, nthreads, [&](int i) {
+        const BlendedBone<R> b = blend_bones<R>(skin + 8 * i, bones);
+        const float* v = rest + 8 * i;
+        R* o = out + 8 * i;
+        for (int c = 0; c < 3; ++c) {
+            o[c] = (((R)v[0] * b.m[0][c] + (R)v[1] * b.m[1][c]) + (R)v[2] * b.m[2][c]) + R(1) * b.m[3][c];
+            o[4 + c] = (((R)v[4] * b.m[0][c] + (R)v[5] * b.m[1][c]) + (R)v[6] * b.m[2][c]) + R(0) * b.m[3][c];
+        }
+        o[3] = (R)v[3];
+        o[7] = (R)v[7];
+    });
+}
+
 extern "C" {
 
 
@@ -905,6 +924,13 @@ void orc_sdf_grid_from_bbox(const float* init_min, const float* init_max, const 
     int tot = (int)(1.4f * dims[0] * dims[1] * dims[2]);
     *total_cells = tot;
     for (int k = 0; k < 3; ++k) origin[k] = cur_min[k] - pad;
+}
+
+int orc32_mesh_skin(const float* rest, const float* skin, int n, const float* bones, float* out, int nthreads) {
+    mesh_skin<float>(rest, skin, n, bones, out, nthreads); return 0;
+}
+int orc64_mesh_skin(const float* rest, const float* skin, int n, const float* bones, double* out, int nthreads) {
+    mesh_skin<double>(rest, skin, n, bones, out, nthreads); return 0;
 }
 
 }  // extern "C"

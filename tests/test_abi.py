@@ -50,6 +50,7 @@ RECORDS = {
     "tfx_sdf_mesh_desc": tfx.SdfMeshDesc,
     "tfx_sdf_collision_params": tfx.SdfCollisionParams,
     "tfx_sdf_grid_info": tfx.SdfGridInfo,
+    "tfx_skinned_mesh_desc": tfx.SkinnedMeshDesc,
 }
 
 

@@ -664,3 +664,61 @@ def test_fp32_and_fp64_transcriptions_agree_on_a_substep():
         widened += w
         p.reset_positions = 0.0 if f >= 1 else 1.0
     assert widened < 0.01 * 12 * 2 * o32.pos.shape[0]
+
+
+def _rand_bone(rng, scale=1.0):
+    """A random rigid bone as a row-major row-vector float4x4 (rotation rows, translation row 3)."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    m = np.eye(4)
+    m[:3, :3] = r.T
+    m[3, :3] = rng.normal(scale=scale, size=3)
+    return m
+
+
+def test_mesh_skin_identity_rigid_blend_and_ignored_slots():
+    """BoneSkinning (TressFXBoneSkinning.hlsl:57-94): identity bones leave the
+    mesh bit-identical; one bone of weight 1 is the rigid transform p·R + t with
+    normals rotated only; unnormalised weights blend the matrices by w/sum(w);
+    slots 1..3 with weight <= 0 are ignored whatever bone they name."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = 500
+    rest = np.zeros((n, 8), np.float32)
+    rest[:, :3] = rng.uniform(-0.2, 0.2, (n, 3))
+    rest[:, 4:7] = rng.uniform(0.1, 1.0, (n, 3))
+    skin = np.zeros((n, 8), np.float32)
+    skin[:, 4] = 1.0
+    eye = np.tile(np.eye(4, dtype=np.float32).reshape(1, 16), (512, 1))
+    out = oracle.mesh_skin(rest, skin, eye)
+    assert np.array_equal(out.view(np.uint32), rest.view(np.uint32))
+
+    bones = np.tile(np.eye(4).reshape(1, 16), (512, 1))
+    b1, b2, b3 = _rand_bone(rng), _rand_bone(rng), _rand_bone(rng, 5.0)
+    bones[1], bones[2], bones[3] = b1.reshape(16), b2.reshape(16), b3.reshape(16)
+    bones32 = bones.astype(np.float32)
+    m1, m2 = bones32[1].astype(np.float64).reshape(4, 4), bones32[2].astype(np.float64).reshape(4, 4)
+    p, nn = rest[:, :3].astype(np.float64), rest[:, 4:7].astype(np.float64)
+    # rigid: bone 1, weight 1
+    skin = np.zeros((n, 8), np.float32)
+    skin[:, 0] = 1
+    skin[:, 4] = 1.0
+    o64 = oracle.mesh_skin(rest, skin, bones32, np.float64)
+    assert np.abs(o64[:, :3] - (p @ m1[:3, :3] + m1[3, :3])).max() < 1e-12
+    assert np.abs(o64[:, 4:7] - nn @ m1[:3, :3]).max() < 1e-12
+    assert np.array_equal(o64[:, 3], rest[:, 3]) and np.array_equal(o64[:, 7], rest[:, 7])
+    # blend: bones 1 and 2 with weights 2 and 6 (normalised by their sum); slot 2
+    # names bone 3 with weight 0 and slot 3 names bone 3 with a negative weight
+    skin = np.zeros((n, 8), np.float32)
+    skin[:, 0], skin[:, 1], skin[:, 2], skin[:, 3] = 1, 2, 3, 3
+    skin[:, 4], skin[:, 5], skin[:, 6], skin[:, 7] = 2.0, 6.0, 0.0, -1.0
+    o64 = oracle.mesh_skin(rest, skin, bones32, np.float64)
+    mb = (2.0 * m1 + 6.0 * m2) / 8.0
+    assert np.abs(o64[:, :3] - (p @ mb[:3, :3] + mb[3, :3])).max() < 1e-12
+    assert np.abs(o64[:, 4:7] - nn @ mb[:3, :3]).max() < 1e-12
+    # the FP32 transcription agrees with FP64 to rounding
+    o32 = oracle.mesh_skin(rest, skin, bones32, np.float32)
+    assert np.abs(o32.astype(np.float64) - o64).max() < 1e-5

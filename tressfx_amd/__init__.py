@@ -179,6 +179,19 @@ class SdfCollisionParams(ctypes.Structure):
     ]
 
 
+class SkinnedMeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_vertices", ctypes.c_int),
+        ("num_triangles", ctypes.c_int),
+        ("positions", c_float_p),
+        ("normals", c_float_p),
+        ("skinning", ctypes.c_void_p),
+        ("indices", ctypes.c_void_p),
+        ("follow_bone", ctypes.c_int),
+        ("pad", ctypes.c_int),
+    ]
+
+
 # Symbols of include/tressfx_amd.h: name -> (restype, argtypes)
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
@@ -220,6 +233,11 @@ EXPORTS = {
     "tfx_sdf_get_data_buffer": (_I, [_VP, ctypes.POINTER(_VP)]),
     "tfx_sdf_download": (_I, [_VP, _VP, c_float_p]),
     "tfx_sdf_get_collision_params": (_I, [_VP, _VP, ctypes.POINTER(SdfCollisionParams)]),
+    "tfx_mesh_skinning_create": (_I, [ctypes.POINTER(SkinnedMeshDesc), ctypes.POINTER(_VP)]),
+    "tfx_mesh_skinning_destroy": (_I, [_VP]),
+    "tfx_mesh_skinning_update": (_I, [_VP, c_float_p, _I, _VP]),
+    "tfx_mesh_skinning_get_mesh": (_I, [_VP, ctypes.POINTER(SdfMeshDesc)]),
+    "tfx_mesh_skinning_download": (_I, [_VP, _VP, _VP]),
     "tfx_batch_create": (_I, [ctypes.POINTER(_VP)]),
     "tfx_batch_destroy": (_I, [_VP]),
     "tfx_batch_add": (_I, [_VP, _VP, _VP]),
@@ -527,6 +545,59 @@ class TressFXSDFCollision:
         g = self.grid()
         out = np.zeros(g.num_cells_x * g.num_cells_y * g.num_cells_z, np.float32)
         _check(self._lib.tfx_sdf_download(self.handle, stream, _fptr(out)))
+        return out
+
+
+class TressFXBoneSkinning:
+    """TressFXBoneSkinning (src/TressFX/TressFXBoneSkinning.h:50-131): a
+    collision mesh skinned on the GPU every Update; `mesh()` is the
+    TressFXSDFInputMeshInterface view an SDF reads.
+    positions/normals float32[n,3], skinning float32[n,8] (engine bone ids as
+    float, then weights), indices uint32[t,3]."""
+
+    def __init__(self, positions, skinning, indices, normals=None, follow_bone=0):
+        self._lib = load_library()
+        self._pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        self._nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        self._skin = np.ascontiguousarray(skinning, np.float32).reshape(-1, 8)
+        self._idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        d = SkinnedMeshDesc()
+        d.num_vertices = self._pos.shape[0]
+        d.num_triangles = self._idx.shape[0]
+        d.positions = _fptr(self._pos)
+        d.normals = _fptr(self._nrm) if self._nrm is not None else None
+        d.skinning = self._skin.ctypes.data
+        d.indices = self._idx.ctypes.data
+        d.follow_bone = int(follow_bone)
+        h = ctypes.c_void_p()
+        _check(self._lib.tfx_mesh_skinning_create(ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h
+        self.num_vertices = d.num_vertices
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_mesh_skinning_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Update(self, bone_matrices, stream=None):
+        m = np.ascontiguousarray(bone_matrices, np.float32).reshape(-1, 16)
+        _check(self._lib.tfx_mesh_skinning_update(self.handle, _fptr(m), m.shape[0], stream))
+
+    def mesh(self):
+        d = SdfMeshDesc()
+        _check(self._lib.tfx_mesh_skinning_get_mesh(self.handle, ctypes.byref(d)))
+        return d
+
+    def download(self, stream=None):
+        """The skinned StandardVertex buffer as float32[n, 8] (synchronous)."""
+        out = np.zeros((self.num_vertices, 8), np.float32)
+        _check(self._lib.tfx_mesh_skinning_download(self.handle, stream, out.ctypes.data))
         return out
 
 

@@ -17,6 +17,7 @@ SOURCES = [
     "tfx_fused_kernel.hip",
     "tfx_sdf_kernels.hip",
     "tfx_batch.hip",
+    "tfx_mesh.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

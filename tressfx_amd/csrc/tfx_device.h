@@ -299,6 +299,16 @@ struct SdfJob {
     SdfGrid g;
 };
 
+// One mesh of a batched skinning pass (TressFXBoneSkinning.hlsl:57-94).
+struct MeshSkinJob {
+    tfx_standard_vertex* out;
+    const tfx_standard_vertex* rest;
+    const tfx_bone_skinning_data* skin;
+    const float* bones;  // row-major row-vector float4x4 per bone
+    int n;
+    int pad;
+};
+
 // GetSdfCoordinates :103-113
 __device__ __forceinline__ void sdf_coords(const SdfGrid& g, f3 p, int& cx, int& cy, int& cz) {
     cx = ftoi((p.x - g.ox) / g.cell);

@@ -213,6 +213,22 @@ struct tfx_sdf {
     int device = 0;
 };
 
+struct tfx_mesh_skinning {
+    int n = 0, ntri = 0, follow_bone = 0;
+    tfx_standard_vertex* rest = nullptr;   // initial vertex positions (StandardVertex, w = 0)
+    tfx_standard_vertex* out = nullptr;    // skinned vertices (the SDF's mesh buffer)
+    tfx_bone_skinning_data* skin = nullptr;
+    uint32_t* idx = nullptr;
+    float* bones_d = nullptr;              // 512 float4x4
+    std::vector<float> bones_h;
+    float bb_min[3] = {0, 0, 0}, bb_max[3] = {0, 0, 0};   // rest bounding cube
+    float cur_min[3] = {0, 0, 0}, cur_max[3] = {0, 0, 0}; // after the last update
+    Staging staging;
+};
+
+extern "C" hipError_t tfx_launch_mesh_skin_multi(const tfx::MeshSkinJob* jobs, int njobs, int max_vertices,
+                                                 hipStream_t stream);
+
 inline void set_identity_bones(std::vector<float>& b) {
     b.assign(16 * TFX_MAX_NUM_BONES, 0.0f);
     for (int i = 0; i < TFX_MAX_NUM_BONES; ++i)
