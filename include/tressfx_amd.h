@@ -371,6 +371,81 @@ int tfx_batch_stage_substep(tfx_batch* batch, const float* bones, int bones_stri
  * membership, then replayed (needs a non-default stream). */
 int tfx_batch_run_frame(tfx_batch* batch, void* stream, int use_graph);
 
+/* ------------------------------------------------------------------------ */
+/* Served form (no counterpart in the reference): a request loop over        */
+/* resident instances with continuous batching.  A worker thread repeatedly   */
+/* advances every instance that has queued frames by one frame in a single   */
+/* batched pass (collision-mesh skinning, SDF builds, fused substeps);        */
+/* requests that arrive while a pass runs join the next one.  Instances are   */
+/* admitted against an HBM byte budget.  Instances belong to the server:     */
+/* their objects may be read (tfx_server_get_objects) only while they have no */
+/* queued or running frames.                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct tfx_server tfx_server;
+typedef struct tfx_server_config {
+    long long hbm_budget_bytes;  /* admission budget for instance state (hair, SDF, mesh buffers) */
+    int max_instances;           /* resident instance limit */
+    int substeps_per_frame;      /* substeps of every frame (1..16) */
+    int device;                  /* HIP device of the worker */
+    int max_inflight;            /* batched passes queued on the GPU at once (1..4) */
+    float substep_dt;            /* time step of every substep (UpdateSimulationParameters' timeStep) */
+    int pad;
+} tfx_server_config;
+typedef struct tfx_instance_desc {
+    const tfx_asset_desc* asset;
+    const tfx_skinned_mesh_desc* mesh;   /* collision mesh, skinned every frame; NULL = no SDF collision */
+    float sdf_origin[3];                 /* explicit SDF grid when sdf_dims[0] > 0 ... */
+    float sdf_cell;
+    int sdf_dims[3];
+    int sdf_num_cells_in_x;              /* ... else sized from the mesh (TressFXSDFCollision.cpp:38-86) */
+    float collision_margin;              /* in cells */
+    int pad;
+    tfx_simulation_settings settings;
+    const float* capsules0;              /* float4[num_capsules] (or NULL) */
+    const float* capsules1;
+    int num_capsules;
+    int pad2;
+} tfx_instance_desc;
+typedef struct tfx_server_stats {
+    long long completed_requests;
+    long long completed_frames;
+    long long vertex_substeps;       /* hair vertices x substeps advanced */
+    long long passes;                /* batched passes launched */
+    double mean_instances_per_pass;
+    double latency_p50_ms;           /* submit -> completion of a request, since the last reset */
+    double latency_p99_ms;
+    double latency_mean_ms;
+    double latency_max_ms;
+    double elapsed_s;                /* wall time since the last reset */
+    long long resident_bytes;
+    int resident_instances;
+    int queued_requests;
+} tfx_server_stats;
+int tfx_server_create(const tfx_server_config* config, tfx_server** out);
+/* Drains queued work, then frees every instance. */
+int tfx_server_destroy(tfx_server* server);
+/* TFX_ERR_ADMISSION if the instance does not fit the budget or the instance
+ * limit now (it may be retried once others retire). */
+int tfx_server_admit(tfx_server* server, const tfx_instance_desc* desc, int* instance_id);
+/* Frees the instance once its queued frames are done; later submits fail. */
+int tfx_server_retire(tfx_server* server, int instance_id);
+/* Queues `frames` frames for an instance, run in submission order:
+ * hair_bones[frames][substeps][num_bones*16] (the hair object's bone
+ * matrices for each substep), mesh_bones[frames][num_mesh_bones*16] (the
+ * collision mesh's, per frame; NULL keeps the current ones),
+ * wind[frames][substeps] (wind magnitude overrides, or NULL). */
+int tfx_server_submit(tfx_server* server, int instance_id, int frames, const float* hair_bones, int num_bones,
+                      const float* mesh_bones, int num_mesh_bones, const float* wind, unsigned long long* ticket);
+/* done = 1 once every frame of the request has completed on the GPU;
+ * latency_ms = completion - submission. */
+int tfx_server_poll(tfx_server* server, unsigned long long ticket, int* done, double* latency_ms);
+int tfx_server_wait(tfx_server* server, unsigned long long ticket, double* latency_ms);
+/* Blocks until nothing is queued or running. */
+int tfx_server_drain(tfx_server* server);
+int tfx_server_stats_read(tfx_server* server, tfx_server_stats* out, int reset);
+/* The instance's hair object and SDF (NULL if none), for inspection while it is idle. */
+int tfx_server_get_objects(tfx_server* server, int instance_id, tfx_hair** hair, tfx_sdf** sdf);
+
 #ifdef __cplusplus
 }
 #endif

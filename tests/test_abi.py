@@ -51,6 +51,9 @@ RECORDS = {
     "tfx_sdf_collision_params": tfx.SdfCollisionParams,
     "tfx_sdf_grid_info": tfx.SdfGridInfo,
     "tfx_skinned_mesh_desc": tfx.SkinnedMeshDesc,
+    "tfx_server_config": tfx.ServerConfig,
+    "tfx_instance_desc": tfx.InstanceDesc,
+    "tfx_server_stats": tfx.ServerStats,
 }
 
 

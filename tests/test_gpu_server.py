@@ -1,0 +1,168 @@
+"""The served form (tfx_server_*): continuous batching over resident
+instances.  Each instance's result is the same alone, in a mix of other
+instances with interleaved requests, and through the per-object reference
+dispatch (bone skinning -> SDF Update -> per substep Simulate + CollideWithHair,
+src/Simulation.cpp:50-131), bit for bit; admission against the HBM budget;
+retirement; error paths."""
+import numpy as np
+import pytest
+
+import tressfx_amd as tfx
+from tressfx_amd import synth
+
+from tfx_testutil import first_mismatch
+
+pytestmark = pytest.mark.gpu
+
+SUBSTEPS = 2
+DT = np.float32(1.0 / synth.FRAME_RATE / SUBSTEPS)
+
+
+def make_instance(i, G, V, grid_n=64, seed=3):
+    wl = synth.config3_instance(i, G, V, grid_n, seed=seed, subdivisions=3)
+    verts = wl.mesh.vertices(0.0)   # static noise; moved by the head bone every frame
+    skin = np.zeros((verts.shape[0], 8), np.float32)
+    skin[:, 4] = 1.0
+    mesh = dict(positions=verts[:, :3], normals=verts[:, 4:7], skinning=skin, indices=wl.mesh.indices, follow_bone=0)
+    return wl, mesh
+
+
+def inputs(wl, f0, frames):
+    hb = np.zeros((frames, SUBSTEPS, 16), np.float32)
+    mb = np.zeros((frames, 16), np.float32)
+    wind = np.zeros((frames, SUBSTEPS), np.float32)
+    for i, f in enumerate(range(f0, f0 + frames)):
+        mb[i] = wl.bone_fn(f / synth.FRAME_RATE).reshape(16)
+        for s in range(SUBSTEPS):
+            t = (f + (s + 1) / SUBSTEPS) / synth.FRAME_RATE
+            hb[i, s] = wl.bone_fn(t).reshape(16)
+            wind[i, s] = wl.wind_fn(t)
+    return hb, mb, wind
+
+
+def admit(server, wl, mesh):
+    return server.admit(wl.asset, wl.settings, mesh=mesh, sdf_grid=wl.sdf_grid, collision_margin=wl.collision_margin)
+
+
+def per_object(wl, mesh, frames):
+    """The same frames through the per-object reference dispatch."""
+    hair = tfx.TressFXHairObject(wl.asset)
+    sk = tfx.TressFXBoneSkinning(mesh["positions"], mesh["skinning"], mesh["indices"], normals=mesh["normals"])
+    origin, cell, dims = wl.sdf_grid
+    sdf = tfx.TressFXSDFCollision(sk.mesh(), collision_margin=wl.collision_margin, explicit=(origin, cell, dims))
+    system = tfx.TressFXSDFCollisionSystem()
+    sim = tfx.TressFXSimulation()
+    hb, mb, wind = inputs(wl, 0, frames)
+    for i in range(frames):
+        sk.Update(mb[i])
+        sdf.SetMesh(sk.mesh())
+        sdf.Update(system)
+        for s in range(SUBSTEPS):
+            hair.UpdateBoneMatrices(hb[i, s])
+            st = tfx.SimulationSettings.from_buffer_copy(wl.settings)
+            st.wind_magnitude = float(wind[i, s])
+            hair.UpdateSimulationParameters(st, DT)
+            sim.Simulate([hair])
+            sdf.CollideWithHair(system, hair)
+    return hair.download_state()
+
+
+def compare(a, b, what):
+    for k in ("pos", "prev", "tangent"):
+        mm = first_mismatch(a[k], b[k])
+        assert mm is None, f"{what}: {k} differs at {mm}"
+
+
+def test_alone_in_mix_and_per_object_are_identical(gpu):
+    print("[server] alone vs mix vs per-object", flush=True)
+    x = make_instance(0, 900, 32)
+    y = make_instance(1, 400, 16)
+    others = [make_instance(2 + k, G, V) for k, (G, V) in enumerate([(1500, 16), (300, 64), (700, 32), (2000, 16)])]
+    frames = 5
+
+    alone = {}
+    for name, (wl, mesh) in (("x", x), ("y", y)):
+        srv = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
+        iid = admit(srv, wl, mesh)
+        hb, mb, wind = inputs(wl, 0, frames)
+        srv.wait(srv.submit(iid, hb, mb, wind))
+        alone[name] = srv.hair(iid).download_state()
+        srv.close()
+
+    srv = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT), max_inflight=2)
+    ids = [admit(srv, wl, mesh) for wl, mesh in others]
+    xid, yid = admit(srv, *x), admit(srv, *y)
+    pos = {i: 0 for i in ids}
+    tickets = []
+
+    def push(i, wl, k):
+        hb, mb, wind = inputs(wl, pos[i], k)
+        pos[i] += k
+        tickets.append(srv.submit(i, hb, mb, wind))
+
+    for i, (wl, _) in zip(ids, others):
+        push(i, wl, 3)
+    hx, mx, wx = inputs(x[0], 0, frames)
+    tickets.append(srv.submit(xid, hx[:2], mx[:2], wx[:2]))      # x joins while others run
+    for i, (wl, _) in zip(ids[:2], others[:2]):
+        push(i, wl, 4)
+    hy, my, wy = inputs(y[0], 0, frames)
+    tickets.append(srv.submit(yid, hy, my, wy))
+    tickets.append(srv.submit(xid, hx[2:], mx[2:], wx[2:]))
+    for i, (wl, _) in zip(ids[2:], others[2:]):
+        push(i, wl, 2)
+    for t in tickets:
+        srv.wait(t)
+    st = srv.stats()
+    print(st)
+    assert st["completed_requests"] == len(tickets)
+    assert st["mean_instances_per_pass"] > 1.5, "requests were not batched"
+    assert st["passes"] < st["completed_frames"]
+    compare(srv.hair(xid).download_state(), alone["x"], "x in the mix vs alone")
+    compare(srv.hair(yid).download_state(), alone["y"], "y in the mix vs alone")
+    compare(alone["x"], per_object(*x, frames), "x served vs per-object dispatch")
+    compare(alone["y"], per_object(*y, frames), "y served vs per-object dispatch")
+    srv.close()
+
+
+def test_admission_budget_and_retire(gpu):
+    print("[server] admission", flush=True)
+    wl, mesh = make_instance(7, 500, 16)
+    probe = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
+    i0 = admit(probe, wl, mesh)
+    need = probe.stats()["resident_bytes"]
+    probe.close()
+    assert need > 0
+    srv = tfx.HairServer(hbm_budget_bytes=int(need * 1.5), substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
+    a = admit(srv, wl, mesh)
+    with pytest.raises(tfx.TFXError) as e:
+        admit(srv, wl, mesh)
+    assert e.value.code == 4
+    hb, mb, wind = inputs(wl, 0, 2)
+    t = srv.submit(a, hb, mb, wind)
+    srv.retire(a)                      # frees once its queued frames are done
+    srv.wait(t)
+    srv.drain()
+    with pytest.raises(tfx.TFXError):
+        srv.submit(a, hb, mb, wind)
+    b = admit(srv, wl, mesh)           # the budget is free again
+    srv.wait(srv.submit(b, hb, mb, wind))
+    assert srv.stats()["resident_instances"] == 1
+    srv.close()
+
+
+def test_server_limits_and_errors(gpu):
+    srv = tfx.HairServer(max_instances=1, substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
+    wl, mesh = make_instance(8, 100, 8)
+    a = admit(srv, wl, mesh)
+    with pytest.raises(tfx.TFXError) as e:
+        admit(srv, wl, mesh)
+    assert e.value.code == 4
+    with pytest.raises(tfx.TFXError):
+        srv.retire(12345)
+    with pytest.raises(tfx.TFXError):
+        srv.poll(999999)
+    hb, mb, wind = inputs(wl, 0, 1)
+    with pytest.raises(tfx.TFXError):
+        srv.submit(a + 100, hb, mb, wind)
+    srv.close()

@@ -192,6 +192,54 @@ class SkinnedMeshDesc(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_longlong),
+        ("max_instances", ctypes.c_int),
+        ("substeps_per_frame", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("max_inflight", ctypes.c_int),
+        ("substep_dt", ctypes.c_float),
+        ("pad", ctypes.c_int),
+    ]
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [
+        ("asset", ctypes.POINTER(AssetDesc)),
+        ("mesh", ctypes.POINTER(SkinnedMeshDesc)),
+        ("sdf_origin", ctypes.c_float * 3),
+        ("sdf_cell", ctypes.c_float),
+        ("sdf_dims", ctypes.c_int * 3),
+        ("sdf_num_cells_in_x", ctypes.c_int),
+        ("collision_margin", ctypes.c_float),
+        ("pad", ctypes.c_int),
+        ("settings", SimulationSettings),
+        ("capsules0", c_float_p),
+        ("capsules1", c_float_p),
+        ("num_capsules", ctypes.c_int),
+        ("pad2", ctypes.c_int),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed_requests", ctypes.c_longlong),
+        ("completed_frames", ctypes.c_longlong),
+        ("vertex_substeps", ctypes.c_longlong),
+        ("passes", ctypes.c_longlong),
+        ("mean_instances_per_pass", ctypes.c_double),
+        ("latency_p50_ms", ctypes.c_double),
+        ("latency_p99_ms", ctypes.c_double),
+        ("latency_mean_ms", ctypes.c_double),
+        ("latency_max_ms", ctypes.c_double),
+        ("elapsed_s", ctypes.c_double),
+        ("resident_bytes", ctypes.c_longlong),
+        ("resident_instances", ctypes.c_int),
+        ("queued_requests", ctypes.c_int),
+    ]
+
+
 # Symbols of include/tressfx_amd.h: name -> (restype, argtypes)
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
@@ -238,6 +286,17 @@ EXPORTS = {
     "tfx_mesh_skinning_update": (_I, [_VP, c_float_p, _I, _VP]),
     "tfx_mesh_skinning_get_mesh": (_I, [_VP, ctypes.POINTER(SdfMeshDesc)]),
     "tfx_mesh_skinning_download": (_I, [_VP, _VP, _VP]),
+    "tfx_server_create": (_I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)]),
+    "tfx_server_destroy": (_I, [_VP]),
+    "tfx_server_admit": (_I, [_VP, ctypes.POINTER(InstanceDesc), ctypes.POINTER(_I)]),
+    "tfx_server_retire": (_I, [_VP, _I]),
+    "tfx_server_submit": (_I, [_VP, _I, _I, c_float_p, _I, c_float_p, _I, c_float_p,
+                               ctypes.POINTER(ctypes.c_ulonglong)]),
+    "tfx_server_poll": (_I, [_VP, ctypes.c_ulonglong, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
+    "tfx_server_wait": (_I, [_VP, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_double)]),
+    "tfx_server_drain": (_I, [_VP]),
+    "tfx_server_stats_read": (_I, [_VP, ctypes.POINTER(ServerStats), _I]),
+    "tfx_server_get_objects": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "tfx_batch_create": (_I, [ctypes.POINTER(_VP)]),
     "tfx_batch_destroy": (_I, [_VP]),
     "tfx_batch_add": (_I, [_VP, _VP, _VP]),
@@ -663,3 +722,137 @@ class HairBatch:
 
     def run_frame(self, stream=None, use_graph=False):
         _check(self._lib.tfx_batch_run_frame(self.handle, stream, 1 if use_graph else 0))
+
+
+class _BorrowedHair(TressFXHairObject):
+    """A hair object owned by someone else (the server): same accessors, no destroy."""
+
+    def __init__(self, handle):
+        self._lib = load_library()
+        self.handle = ctypes.c_void_p(handle)
+        info = self.info()
+        self.num_guide_strands = info.num_guide_strands
+        self.num_follow = info.num_follow_strands_per_guide
+        self.V = info.num_vertices_per_strand
+
+    def close(self):
+        self.handle = None
+
+
+class HairServer:
+    """The served form (tfx_server_*): continuous batching over resident instances."""
+
+    def __init__(self, hbm_budget_bytes=200 << 30, max_instances=1000, substeps_per_frame=2,
+                 substep_dt=1.0 / 120.0, max_inflight=2, device=0):
+        self._lib = load_library()
+        c = ServerConfig()
+        c.hbm_budget_bytes = int(hbm_budget_bytes)
+        c.max_instances = int(max_instances)
+        c.substeps_per_frame = int(substeps_per_frame)
+        c.device = int(device)
+        c.max_inflight = int(max_inflight)
+        c.substep_dt = float(substep_dt)
+        h = ctypes.c_void_p()
+        _check(self._lib.tfx_server_create(ctypes.byref(c), ctypes.byref(h)))
+        self.handle = h
+        self.substeps = int(substeps_per_frame)
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, asset, settings, mesh=None, sdf_grid=None, sdf_num_cells_in_x=0, collision_margin=0.5,
+              capsules=None):
+        """mesh: dict(positions, skinning, indices, normals=None, follow_bone=0);
+        sdf_grid: (origin, cell, dims) explicit grid, else sdf_num_cells_in_x.
+        Raises TFXError (code TFX_ERR_ADMISSION = 4) when it does not fit now."""
+        d = InstanceDesc()
+        ad = asset.desc()
+        d.asset = ctypes.pointer(ad)
+        keep = [ad, asset]
+        if mesh is not None:
+            pos = np.ascontiguousarray(mesh["positions"], np.float32).reshape(-1, 3)
+            nrm = mesh.get("normals")
+            nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+            sk = np.ascontiguousarray(mesh["skinning"], np.float32).reshape(-1, 8)
+            idx = np.ascontiguousarray(mesh["indices"], np.uint32).reshape(-1, 3)
+            md = SkinnedMeshDesc()
+            md.num_vertices, md.num_triangles = pos.shape[0], idx.shape[0]
+            md.positions = _fptr(pos)
+            md.normals = _fptr(nrm) if nrm is not None else None
+            md.skinning = sk.ctypes.data
+            md.indices = idx.ctypes.data
+            md.follow_bone = int(mesh.get("follow_bone", 0))
+            d.mesh = ctypes.pointer(md)
+            keep += [pos, nrm, sk, idx, md]
+            if sdf_grid is not None:
+                origin, cell, dims = sdf_grid
+                for k in range(3):
+                    d.sdf_origin[k] = float(origin[k])
+                    d.sdf_dims[k] = int(dims[k])
+                d.sdf_cell = float(cell)
+            else:
+                d.sdf_num_cells_in_x = int(sdf_num_cells_in_x)
+            d.collision_margin = float(collision_margin)
+        d.settings = settings
+        if capsules is not None:
+            c0 = np.ascontiguousarray(capsules[0], np.float32).reshape(-1, 4)
+            c1 = np.ascontiguousarray(capsules[1], np.float32).reshape(-1, 4)
+            d.capsules0, d.capsules1, d.num_capsules = _fptr(c0), _fptr(c1), c0.shape[0]
+            keep += [c0, c1]
+        out = ctypes.c_int()
+        _check(self._lib.tfx_server_admit(self.handle, ctypes.byref(d), ctypes.byref(out)))
+        return out.value
+
+    def retire(self, instance_id):
+        _check(self._lib.tfx_server_retire(self.handle, int(instance_id)))
+
+    def submit(self, instance_id, hair_bones, mesh_bones=None, wind=None):
+        """hair_bones float32[frames, substeps, nb, 16] (or [frames, substeps, 16]);
+        mesh_bones float32[frames, nmb, 16] or None; wind float32[frames, substeps] or None."""
+        hb = np.ascontiguousarray(hair_bones, np.float32)
+        frames = hb.shape[0]
+        hb = hb.reshape(frames, self.substeps, -1)
+        nb = hb.shape[2] // 16
+        mb, nmb = None, 0
+        if mesh_bones is not None:
+            mb = np.ascontiguousarray(mesh_bones, np.float32).reshape(frames, -1)
+            nmb = mb.shape[1] // 16
+        w = None if wind is None else np.ascontiguousarray(wind, np.float32).reshape(frames, self.substeps)
+        t = ctypes.c_ulonglong()
+        _check(self._lib.tfx_server_submit(self.handle, int(instance_id), frames, _fptr(hb), nb,
+                                           _fptr(mb) if mb is not None else None, nmb,
+                                           _fptr(w) if w is not None else None, ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        done, lat = ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.tfx_server_poll(self.handle, ctypes.c_ulonglong(ticket), ctypes.byref(done),
+                                         ctypes.byref(lat)))
+        return bool(done.value), lat.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double()
+        _check(self._lib.tfx_server_wait(self.handle, ctypes.c_ulonglong(ticket), ctypes.byref(lat)))
+        return lat.value
+
+    def drain(self):
+        _check(self._lib.tfx_server_drain(self.handle))
+
+    def stats(self, reset=False):
+        st = ServerStats()
+        _check(self._lib.tfx_server_stats_read(self.handle, ctypes.byref(st), 1 if reset else 0))
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def hair(self, instance_id):
+        """The instance's hair object (borrowed; the instance must be idle)."""
+        h, sdf = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(self._lib.tfx_server_get_objects(self.handle, int(instance_id), ctypes.byref(h), ctypes.byref(sdf)))
+        return _BorrowedHair(h.value)

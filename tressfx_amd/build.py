@@ -18,6 +18,7 @@ SOURCES = [
     "tfx_sdf_kernels.hip",
     "tfx_batch.hip",
     "tfx_mesh.hip",
+    "tfx_server.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -10,9 +10,6 @@
 // CollideWithHair (each member collides with its own SDF).
 #include "tfx_internal.h"
 
-extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
-                                           hipStream_t stream);
-
 namespace {
 
 constexpr int kBatchMaxSubsteps = 16;

@@ -4,7 +4,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -225,17 +224,8 @@ extern "C" int tfx_hair_create(const tfx_asset_desc* a, tfx_hair** out) {
     HIP_TRY(hipMemcpy(h->bones_d, h->bones_h.data(), h->bones_h.size() * sizeof(float), hipMemcpyHostToDevice));
     h->bones_dirty = 0;
     compute_tangent_w(h, a->tangents);
-    // The matrices the kernels can read: index ftoi(bone_index) clamped to
-    // [0, 511] for slot 0 and every slot of positive weight (blend_bones).
-    h->bones_used = 1;
-    for (long long st = 0; st < S; st += (h->F + 1))
-        for (int k = 0; k < 4; ++k)
-            if (k == 0 || a->bone_skinning_data[st].bone_weight[k] > 0.0f) {
-                const float fi = a->bone_skinning_data[st].bone_index[k];
-                int idx = !(fi == fi) ? 0 : fi >= 2147483648.0f ? 2147483647 : fi <= -2147483648.0f ? INT_MIN : (int)fi;
-                idx = std::max(0, std::min(idx, TFX_MAX_NUM_BONES - 1));
-                h->bones_used = std::max(h->bones_used, idx + 1);
-            }
+    // The matrices the kernels can read (guide strands' skinning records).
+    h->bones_used = skin_bones_used(a->bone_skinning_data, S, h->F + 1);
     // Constant block before the first UpdateSimulationParameters: zero, with
     // the counts of the object and the reset flag of the constructor (:157).
     std::memset(&h->params, 0, sizeof(h->params));

@@ -221,6 +221,7 @@ struct tfx_mesh_skinning {
     uint32_t* idx = nullptr;
     float* bones_d = nullptr;              // 512 float4x4
     std::vector<float> bones_h;
+    int bones_used = 1;                    // 1 + the largest bone index the skinning data can read
     float bb_min[3] = {0, 0, 0}, bb_max[3] = {0, 0, 0};   // rest bounding cube
     float cur_min[3] = {0, 0, 0}, cur_max[3] = {0, 0, 0}; // after the last update
     Staging staging;
@@ -228,6 +229,39 @@ struct tfx_mesh_skinning {
 
 extern "C" hipError_t tfx_launch_mesh_skin_multi(const tfx::MeshSkinJob* jobs, int njobs, int max_vertices,
                                                  hipStream_t stream);
+extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
+                                           hipStream_t stream);
+
+// The bounding box that follows the follow bone (TressFXBoneSkinning::
+// GetBoundingBox, TressFXBoneSkinning.cpp:439-452): the rest cube moved by the
+// translation the bone matrix gives its centre.
+inline void mesh_follow_bbox(tfx_mesh_skinning* m) {
+    const float* b = &m->bones_h[16 * m->follow_bone];
+    float c[3], nc[3];
+    for (int k = 0; k < 3; ++k) c[k] = (m->bb_max[k] + m->bb_min[k]) * 0.5f;
+    for (int k = 0; k < 3; ++k) nc[k] = ((c[0] * b[k] + c[1] * b[4 + k]) + c[2] * b[8 + k]) + b[12 + k];
+    for (int k = 0; k < 3; ++k) {
+        m->cur_min[k] = m->bb_min[k] + (nc[k] - c[k]);
+        m->cur_max[k] = m->bb_max[k] + (nc[k] - c[k]);
+    }
+}
+
+// Bone table entries a skinning record can read: slot 0 always, slots 1..3
+// when their weight is positive, index ftoi(bone_index) clamped to [0, 511]
+// (bone_ptr).  Returns 1 + the largest.
+inline int skin_bones_used(const tfx_bone_skinning_data* sd, long long count, long long stride) {
+    int used = 1;
+    for (long long i = 0; i < count; i += stride)
+        for (int k = 0; k < 4; ++k)
+            if (k == 0 || sd[i].bone_weight[k] > 0.0f) {
+                const float fi = sd[i].bone_index[k];
+                int idx = !(fi == fi) ? 0 : fi >= 2147483648.0f ? 2147483647 : fi <= -2147483648.0f ? (-2147483647 - 1)
+                                                                                                     : (int)fi;
+                idx = std::max(0, std::min(idx, TFX_MAX_NUM_BONES - 1));
+                used = std::max(used, idx + 1);
+            }
+    return used;
+}
 
 inline void set_identity_bones(std::vector<float>& b) {
     b.assign(16 * TFX_MAX_NUM_BONES, 0.0f);

@@ -97,6 +97,7 @@ extern "C" int tfx_mesh_skinning_create(const tfx_skinned_mesh_desc* d, tfx_mesh
     m->n = n;
     m->ntri = d->num_triangles;
     m->follow_bone = std::min(std::max(d->follow_bone, 0), TFX_MAX_NUM_BONES - 1);
+    m->bones_used = std::max(skin_bones_used(d->skinning, n, 1), m->follow_bone + 1);
     std::vector<tfx_standard_vertex> rest(n);
     for (int i = 0; i < n; ++i) {
         std::memset(&rest[i], 0, sizeof(tfx_standard_vertex));
@@ -155,27 +156,13 @@ extern "C" int tfx_mesh_skinning_destroy(tfx_mesh_skinning* m) {
     return TFX_OK;
 }
 
-// The bounding box that follows the follow bone (GetBoundingBox,
-// TressFXBoneSkinning.cpp:439-452): the rest cube moved by the translation
-// the bone matrix gives its centre.
-static void follow_bbox(tfx_mesh_skinning* m) {
-    const float* b = &m->bones_h[16 * m->follow_bone];
-    float c[3], nc[3];
-    for (int k = 0; k < 3; ++k) c[k] = (m->bb_max[k] + m->bb_min[k]) * 0.5f;
-    for (int k = 0; k < 3; ++k) nc[k] = ((c[0] * b[k] + c[1] * b[4 + k]) + c[2] * b[8 + k]) + b[12 + k];
-    for (int k = 0; k < 3; ++k) {
-        m->cur_min[k] = m->bb_min[k] + (nc[k] - c[k]);
-        m->cur_max[k] = m->bb_max[k] + (nc[k] - c[k]);
-    }
-}
-
 // TressFXBoneSkinning::Update, TressFXBoneSkinning.cpp:460-504.
 extern "C" int tfx_mesh_skinning_update(tfx_mesh_skinning* m, const float* bones, int num_bones, void* stream) {
     CHECK_ARG(m && (bones || num_bones == 0) && num_bones >= 0, "tfx_mesh_skinning_update: bad argument");
     hipStream_t st = (hipStream_t)stream;
     const int nb = std::min(num_bones, TFX_MAX_NUM_BONES);
     if (nb) std::memcpy(m->bones_h.data(), bones, (size_t)nb * 16 * sizeof(float));
-    follow_bbox(m);
+    mesh_follow_bbox(m);
     int rc = m->staging.ensure(16 * TFX_MAX_NUM_BONES * sizeof(float));
     if (rc) return rc;
     void* host;
