@@ -16,7 +16,8 @@ The timed region replays the frame as a hipGraph through the batched form
 the roofline of the dominant kernel (the fused substep; its duration from HIP
 events around each launch in a second, directly launched leg), the plain
 per-instance dispatch for comparison, the 256-instance batched workload
-against its plain dispatch, and the CPU transcription's throughput.
+against its plain dispatch, the served form (continuous batching over arriving
+instances and step requests) and the CPU transcription's throughput.
 """
 import argparse
 import json
@@ -116,6 +117,9 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=20, help="timed frames of the 256-instance workload")
     ap.add_argument("--batched-plain-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--served-seconds", type=float, default=12.0,
+                    help="length of the served-form request window (0: skip)")
+    ap.add_argument("--served-initial", type=int, default=256, help="instances resident when the window opens")
     ap.add_argument("--cpu-frames", type=int, default=600)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -246,6 +250,32 @@ def main():
             "speedup_vs_plain": (bval / pval) if pval else None,
         }
 
+    served = None
+    if wls3 is not None and args.served_seconds > 0:
+        srv = tfx.HairServer(hbm_budget_bytes=200 << 30, max_instances=1000, substeps_per_frame=2,
+                             substep_dt=float(np.float32(1.0 / synth.FRAME_RATE / 2)), max_inflight=2)
+        run = harness.ServedRun(srv, wls3, initial=args.served_initial, duration_s=args.served_seconds,
+                                inst_rate=20.0, lifetime_s=30.0, req_rate=5000.0, ks=(1, 2, 4))
+        _log(f"served window {args.served_seconds:.0f} s, {args.served_initial} instances resident at start")
+        res = run.run(log=_log)
+        srv.close()
+        sv = res["vertex_substeps"] / res["elapsed_s"]
+        served = {
+            "workload": "config4: instances drawn from the 256 config-3 instances, Poisson arrivals 20/s, "
+                        "lifetimes Exp(30 s), step requests Poisson 5000/s of k in {1,2,4} frames on random "
+                        f"resident instances, 200 GB admission budget, {args.served_initial} resident at start, "
+                        f"{args.served_seconds:.0f} s window",
+            "value": sv, "unit": "vertex-substeps/s",
+            "latency_ms": {"p50": res["latency_p50_ms"], "p99": res["latency_p99_ms"], "mean": res["latency_mean_ms"],
+                           "max": res["latency_max_ms"]},
+            "requests_completed": res["completed_requests"], "frames_completed": res["completed_frames"],
+            "offered_request_rate": res["offered_request_rate"], "passes": res["passes"],
+            "mean_instances_per_pass": res["mean_instances_per_pass"], "window_s": res["elapsed_s"],
+            "instances_retired": res["instances_retired"], "admissions_refused": res["admissions_refused"],
+            "resident_at_end": res["resident_at_end"],
+            "speedup_vs_plain": (sv / batched["plain"]["value"]) if batched and batched.get("plain") else None,
+        }
+
     cpu = None
     if not args.no_cpu:
         cwl = synth.config2(num_guides=args.guides)
@@ -295,6 +325,7 @@ def main():
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -433,7 +433,8 @@ int tfx_server_retire(tfx_server* server, int instance_id);
  * hair_bones[frames][substeps][num_bones*16] (the hair object's bone
  * matrices for each substep), mesh_bones[frames][num_mesh_bones*16] (the
  * collision mesh's, per frame; NULL keeps the current ones),
- * wind[frames][substeps] (wind magnitude overrides, or NULL). */
+ * wind[frames][substeps] (wind magnitude overrides, or NULL).  ticket NULL:
+ * nothing to poll; the request only counts in the statistics. */
 int tfx_server_submit(tfx_server* server, int instance_id, int frames, const float* hair_bones, int num_bones,
                       const float* mesh_bones, int num_mesh_bones, const float* wind, unsigned long long* ticket);
 /* done = 1 once every frame of the request has completed on the GPU;

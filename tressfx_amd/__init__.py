@@ -814,7 +814,7 @@ class HairServer:
     def retire(self, instance_id):
         _check(self._lib.tfx_server_retire(self.handle, int(instance_id)))
 
-    def submit(self, instance_id, hair_bones, mesh_bones=None, wind=None):
+    def submit(self, instance_id, hair_bones, mesh_bones=None, wind=None, track=True):
         """hair_bones float32[frames, substeps, nb, 16] (or [frames, substeps, 16]);
         mesh_bones float32[frames, nmb, 16] or None; wind float32[frames, substeps] or None."""
         hb = np.ascontiguousarray(hair_bones, np.float32)
@@ -829,8 +829,9 @@ class HairServer:
         t = ctypes.c_ulonglong()
         _check(self._lib.tfx_server_submit(self.handle, int(instance_id), frames, _fptr(hb), nb,
                                            _fptr(mb) if mb is not None else None, nmb,
-                                           _fptr(w) if w is not None else None, ctypes.byref(t)))
-        return t.value
+                                           _fptr(w) if w is not None else None,
+                                           ctypes.byref(t) if track else None))
+        return t.value if track else None
 
     def poll(self, ticket):
         done, lat = ctypes.c_int(), ctypes.c_double()

@@ -35,6 +35,7 @@ struct Request {
     std::vector<float> mesh_bones;  // [frames][nmb * 16] (empty: keep)
     std::vector<float> wind;        // [frames][substeps] (empty: settings)
     double t_submit = 0.0;
+    bool keep_result = true;  // false: counted in the statistics only
 };
 
 struct Instance {
@@ -154,9 +155,11 @@ void retire_pass(tfx_server* s) {
         r->done += 1;
         if (r->done == r->frames) {
             const double ms = (t - r->t_submit) * 1e3;
-            Result& res = s->results[r->ticket];
-            res.finished = true;
-            res.latency_ms = ms;
+            if (r->keep_result) {
+                Result& res = s->results[r->ticket];
+                res.finished = true;
+                res.latency_ms = ms;
+            }
             s->latencies.push_back(ms);
             s->completed_requests += 1;
             s->unfinished -= 1;
@@ -185,9 +188,11 @@ void fail_all(tfx_server* s, int code, const std::string& msg) {
     s->error_msg = msg;
     for (auto& kv : s->instances) {
         for (Request* r : kv.second->queue) {
-            Result& res = s->results[r->ticket];
-            res.finished = true;
-            res.status = code;
+            if (r->keep_result) {
+                Result& res = s->results[r->ticket];
+                res.finished = true;
+                res.status = code;
+            }
             s->unfinished -= 1;
             delete r;
         }
@@ -579,7 +584,7 @@ extern "C" int tfx_server_retire(tfx_server* s, int id) {
 extern "C" int tfx_server_submit(tfx_server* s, int id, int frames, const float* hair_bones, int num_bones,
                                  const float* mesh_bones, int num_mesh_bones, const float* wind,
                                  unsigned long long* ticket) {
-    CHECK_ARG(s && ticket && frames > 0, "tfx_server_submit: bad argument");
+    CHECK_ARG(s && frames > 0, "tfx_server_submit: bad argument");
     CHECK_ARG(num_bones >= 0 && num_mesh_bones >= 0 && (hair_bones || num_bones == 0) &&
                   (mesh_bones || num_mesh_bones == 0),
               "tfx_server_submit: bad bone arguments");
@@ -604,10 +609,13 @@ extern "C" int tfx_server_submit(tfx_server* s, int id, int frames, const float*
         }
         r->ticket = s->next_ticket++;
         r->t_submit = now_s();
+        r->keep_result = ticket != nullptr;
         it->second->queue.push_back(r);
-        s->results[r->ticket] = Result{};
+        if (ticket) {
+            s->results[r->ticket] = Result{};
+            *ticket = r->ticket;
+        }
         s->unfinished += 1;
-        *ticket = r->ticket;
     }
     s->cv_work.notify_all();
     return TFX_OK;

@@ -220,3 +220,145 @@ class PlainMultiDriver:
             for hair, sdf in zip(self.hairs, self.sdfs):
                 if sdf is not None:
                     sdf.CollideWithHair(self.system, hair, stream)
+
+
+def served_mesh(wl):
+    """The collision mesh of a workload as the served form takes it: the rest
+    mesh (static noise) skinned every frame by the head bone (bone 0)."""
+    verts = wl.mesh.vertices(0.0)
+    skin = np.zeros((verts.shape[0], 8), np.float32)
+    skin[:, 4] = 1.0
+    return dict(positions=verts[:, :3], normals=verts[:, 4:7], skinning=skin, indices=wl.mesh.indices, follow_bone=0)
+
+
+class ServedRun:
+    """A client of the served form: instances arriving as a Poisson process
+    (each drawn from `workloads`, so the sizes follow their distribution) with
+    exponential lifetimes, and step requests of k frames (k uniform over `ks`)
+    arriving as a Poisson process on random resident instances.  A refused
+    admission waits in a FIFO and is retried as memory frees up.  Per-frame
+    inputs (bones for every substep, the collision mesh's bone, wind) are
+    evaluated for `horizon` frames per workload before the run."""
+
+    def __init__(self, server, workloads, initial, duration_s, inst_rate=20.0, lifetime_s=30.0, req_rate=5000.0,
+                 ks=(1, 2, 4), seed=4, horizon=512):
+        self.server = server
+        self.wls = list(workloads)
+        self.initial = int(initial)
+        self.duration = float(duration_s)
+        self.ks = tuple(ks)
+        self.substeps = server.substeps
+        self.horizon = int(horizon)
+        rng = np.random.Generator(np.random.PCG64(seed))
+        self.rng = rng
+        # arrivals and requests on [0, duration)
+        n_arr = rng.poisson(inst_rate * self.duration)
+        self.arrivals = np.sort(rng.uniform(0.0, self.duration, n_arr))
+        self.arrival_wl = rng.integers(0, len(self.wls), n_arr)
+        n_req = rng.poisson(req_rate * self.duration)
+        self.requests = np.sort(rng.uniform(0.0, self.duration, n_req))
+        self.req_k = rng.choice(self.ks, n_req)
+        self.lifetime_s = float(lifetime_s)
+        # per-workload inputs for `horizon` frames
+        sub = self.substeps
+        self.inputs = []
+        for wl in self.wls:
+            f = np.arange(self.horizon, dtype=np.float64)
+            t_sub = (f[:, None] + (np.arange(sub)[None, :] + 1.0) / sub) / synth.FRAME_RATE
+            hb = synth.trajectory_bones(wl.bone_fn, t_sub)                # [H, sub, 16]
+            mb = synth.trajectory_bones(wl.bone_fn, f / synth.FRAME_RATE)  # [H, 16]
+            wind = (np.vectorize(wl.wind_fn)(t_sub) if wl.wind_fn is not None
+                    else np.full(t_sub.shape, wl.settings.wind_magnitude)).astype(np.float32)
+            self.inputs.append((np.ascontiguousarray(hb), np.ascontiguousarray(mb), np.ascontiguousarray(wind)))
+        self.meshes = [served_mesh(wl) for wl in self.wls]
+
+    def _admit(self, w):
+        wl = self.wls[w]
+        try:
+            return self.server.admit(wl.asset, wl.settings, mesh=self.meshes[w], sdf_grid=wl.sdf_grid,
+                                     collision_margin=wl.collision_margin, capsules=wl.capsules)
+        except Exception as e:  # noqa: BLE001 - admission refusals come back as TFXError code 4
+            if getattr(e, "code", None) == 4:
+                return None
+            raise
+
+    def run(self, log=None):
+        import time
+        srv = self.server
+        resident = []          # instance ids
+        info = {}              # id -> [workload, next frame, departure time]
+        refused = 0
+        pending = []           # (workload) waiting for admission
+        for k in range(self.initial):
+            w = k % len(self.wls)
+            iid = self._admit(w)
+            if iid is None:
+                refused += 1
+                pending.append(w)
+                continue
+            resident.append(iid)
+            info[iid] = [w, 0, self.rng.exponential(self.lifetime_s)]
+        srv.stats(reset=True)
+        t0 = time.perf_counter()
+        ia = ir = 0
+        submitted = frames = retired = 0
+        next_retry = 0.0
+        last_log = 0.0
+        while ia < len(self.arrivals) or ir < len(self.requests):
+            now = time.perf_counter() - t0
+            ta = self.arrivals[ia] if ia < len(self.arrivals) else np.inf
+            tr = self.requests[ir] if ir < len(self.requests) else np.inf
+            due = min(ta, tr)
+            if due > now:
+                time.sleep(min(due - now, 0.002))
+                continue
+            # departures
+            if resident and now >= next_retry:
+                for iid in [i for i in resident if info[i][2] <= now]:
+                    srv.retire(iid)
+                    resident.remove(iid)
+                    del info[iid]
+                    retired += 1
+                while pending:
+                    iid = self._admit(pending[0])
+                    if iid is None:
+                        break
+                    w = pending.pop(0)
+                    resident.append(iid)
+                    info[iid] = [w, 0, now + self.rng.exponential(self.lifetime_s)]
+                next_retry = now + 0.05
+            if ta <= tr:
+                w = int(self.arrival_wl[ia])
+                ia += 1
+                iid = self._admit(w)
+                if iid is None:
+                    refused += 1
+                    pending.append(w)
+                else:
+                    resident.append(iid)
+                    info[iid] = [w, 0, now + self.rng.exponential(self.lifetime_s)]
+            else:
+                k = int(self.req_k[ir])
+                ir += 1
+                if not resident:
+                    continue
+                iid = resident[int(self.rng.integers(0, len(resident)))]
+                w, f, _ = info[iid]
+                hb, mb, wind = self.inputs[w]
+                sl = np.arange(f, f + k) % self.horizon
+                srv.submit(iid, hb[sl], mb[sl], wind[sl], track=False)
+                info[iid][1] = f + k
+                submitted += 1
+                frames += k
+            if log and now - last_log > 1.0:
+                last_log = now
+                log(f"served t={now:.1f}s resident={len(resident)} submitted={submitted}")
+        t_sub = time.perf_counter() - t0
+        srv.drain()
+        t1 = time.perf_counter() - t0
+        st = srv.stats()
+        return {
+            "window_s": t1, "submit_window_s": t_sub, "requests_submitted": submitted, "frames_submitted": frames,
+            "offered_request_rate": submitted / t_sub if t_sub > 0 else 0.0, "instances_retired": retired,
+            "admissions_refused": refused, "resident_at_end": len(resident), **st,
+        }

@@ -331,7 +331,35 @@ def rigid_trajectory(rng):
     def bone(t):
         return yaw_pitch_matrix(amp[0] * math.sin(2.0 * math.pi * hz[0] * t + ph[0]),
                                 amp[1] * math.sin(2.0 * math.pi * hz[1] * t + ph[1]))
+    bone.params = (amp, hz, ph)
     return bone
+
+
+def yaw_pitch_matrices(yaw, pitch):
+    """yaw_pitch_matrix over arrays of angles: float32[T, 16]."""
+    yaw, pitch = np.asarray(yaw, np.float64), np.asarray(pitch, np.float64)
+    cy, sy, cp, sp = np.cos(yaw), np.sin(yaw), np.cos(pitch), np.sin(pitch)
+    z = np.zeros_like(cy)
+    # r = R_y R_x (column-vector); the row-vector matrix holds r transposed
+    r = np.stack([np.stack([cy, sy * sp, sy * cp], -1), np.stack([z, cp, -sp], -1),
+                  np.stack([-sy, cy * sp, cy * cp], -1)], -2)
+    m = np.zeros(yaw.shape + (4, 4))
+    m[..., :3, :3] = np.swapaxes(r, -1, -2)
+    m[..., 3, 3] = 1.0
+    return m.reshape(yaw.shape + (16,)).astype(np.float32)
+
+
+def trajectory_bones(bone_fn, times):
+    """bone_fn at every time of `times` (float32[..., 16]); vectorised for
+    rigid_trajectory and head_yaw."""
+    times = np.asarray(times, np.float64)
+    params = getattr(bone_fn, "params", None)
+    if params is not None:
+        amp, hz, ph = params
+        return yaw_pitch_matrices(amp[0] * np.sin(2.0 * np.pi * hz[0] * times + ph[0]),
+                                  amp[1] * np.sin(2.0 * np.pi * hz[1] * times + ph[1]))
+    flat = np.stack([np.asarray(bone_fn(float(t)), np.float32).reshape(16) for t in times.reshape(-1)])
+    return flat.reshape(times.shape + (16,))
 
 
 CONFIG3_LENGTHS = (16, 32, 64)
