@@ -60,6 +60,20 @@ __device__ __forceinline__ TriPrep tri_prep(f3 x0, f3 x1, f3 x2) {
     return t;
 }
 
+// The squared length DistancePointToEdge takes the square root of.
+__device__ __forceinline__ float sq_distance_point_to_edge(f3 p, f3 x0, f3 x1) {
+    f3 x10 = sub(x1, x0);
+    float t = dot3(sub(x1, p), x10) / dot3(x10, x10);
+    t = fmx(0.0f, fmn(t, 1.0f));
+    f3 a = sub(p, add(mul(t, x0), mul(1.0f - t, x1)));
+    return dot3(a, a);
+}
+
+// SignedDistancePointToTriangle :163-215.  Outside the face region the
+// reference takes min(sqrt(e01), sqrt(e12), sqrt(e02)) of the three squared
+// edge distances; the correctly rounded square root is monotone, so
+// sqrt(min(e01, e12, e02)) is the same value (also for infinities and NaNs,
+// which fmin drops) with one square root instead of three.
 __device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p) {
     f3 px2 = sub(p, t.x2);
     float b = dot3(t.x12n, px2) / t.l2;
@@ -71,13 +85,52 @@ __device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p) {
         f3 n = sub(p, add(add(mul(a, t.x0), mul(b, t.x1)), mul(c, t.x2)));
         d = len3(n);
     } else {
-        d = distance_point_to_edge(p, t.x0, t.x1);
-        float d12 = distance_point_to_edge(p, t.x1, t.x2);
-        float d02 = distance_point_to_edge(p, t.x0, t.x2);
-        d = fmn(d, d12);
-        d = fmn(d, d02);
+        float e = sq_distance_point_to_edge(p, t.x0, t.x1);
+        e = fmn(e, sq_distance_point_to_edge(p, t.x1, t.x2));
+        e = fmn(e, sq_distance_point_to_edge(p, t.x0, t.x2));
+        d = sqrtf(e);
     }
     return (dot3(sub(p, t.x0), t.nTri) < 0.0f) ? -d : d;
+}
+
+// i / n and i % n for 0 <= i < 2^24, n >= 1: the float estimate is within one
+// of the quotient, and one correction step makes it exact.
+__device__ __forceinline__ void divmod24(int i, int n, float inv_n, int& q, int& r) {
+    q = (int)((float)i * inv_n);
+    r = i - q * n;
+    if (r < 0) {
+        q -= 1;
+        r += n;
+    } else if (r >= n) {
+        q += 1;
+        r -= n;
+    }
+}
+
+// Finalize of one node: IFloatFlip3, and a node closer than 1e10 marks every
+// brick whose kBrickHalo-dilated extent contains it.
+__device__ __forceinline__ void finalize_cell(uint32_t* __restrict__ cells, int i, const SdfGrid& g, float inv_nx,
+                                              float inv_ny, bool fast_index) {
+    const uint32_t u = cells[i];
+    cells[i] = ifloat_flip3(u);
+    if (u == float_flip3(1e10f) || !g.bricks) return;
+    int x, y, z, r;
+    if (fast_index) {
+        divmod24(i, g.nx, inv_nx, r, x);
+        divmod24(r, g.ny, inv_ny, z, y);
+    } else {
+        x = i % g.nx;
+        r = i / g.nx;
+        y = r % g.ny;
+        z = r / g.ny;
+    }
+    const int x0 = max(x - kBrickHalo, 0) >> kBrickShift, x1 = min((x + kBrickHalo) >> kBrickShift, g.bx - 1);
+    const int y0 = max(y - kBrickHalo, 0) >> kBrickShift, y1 = min((y + kBrickHalo) >> kBrickShift, g.by - 1);
+    const int z0 = max(z - kBrickHalo, 0) >> kBrickShift, z1 = min((z + kBrickHalo) >> kBrickShift, g.bz - 1);
+    uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
+    for (int bz = z0; bz <= z1; ++bz)
+        for (int by = y0; by <= y1; ++by)
+            for (int bx = x0; bx <= x1; ++bx) bricks[(bz * g.by + by) * g.bx + bx] = 1;
 }
 
 // InitializeSignedDistanceField :301-312, plus clearing the brick occupancy.
@@ -91,22 +144,11 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_init(uint32_t* __restrict__ s
 
 // FinalizeSignedDistanceField :400-412.  A cell that ends up closer than 1e10
 // also marks every brick whose kBrickHalo-dilated extent contains it.
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize(uint32_t* __restrict__ sdf, int ncell, SdfGrid g,
-                                                            uint8_t* __restrict__ bricks) {
-    const uint32_t far = float_flip3(1e10f);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock) {
-        const uint32_t u = sdf[i];
-        sdf[i] = ifloat_flip3(u);
-        if (bricks && u != far) {
-            const int x = i % g.nx, r = i / g.nx, y = r % g.ny, z = r / g.ny;
-            const int x0 = max(x - kBrickHalo, 0) >> kBrickShift, x1 = min((x + kBrickHalo) >> kBrickShift, g.bx - 1);
-            const int y0 = max(y - kBrickHalo, 0) >> kBrickShift, y1 = min((y + kBrickHalo) >> kBrickShift, g.by - 1);
-            const int z0 = max(z - kBrickHalo, 0) >> kBrickShift, z1 = min((z + kBrickHalo) >> kBrickShift, g.bz - 1);
-            for (int bz = z0; bz <= z1; ++bz)
-                for (int by = y0; by <= y1; ++by)
-                    for (int bx = x0; bx <= x1; ++bx) bricks[(bz * g.by + by) * g.bx + bx] = 1;
-        }
-    }
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize(uint32_t* __restrict__ sdf, int ncell, SdfGrid g) {
+    const float inv_nx = 1.0f / (float)g.nx, inv_ny = 1.0f / (float)g.ny;
+    const bool fast = ncell <= (1 << 24);
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock)
+        finalize_cell(sdf, i, g, inv_nx, inv_ny, fast);
 }
 
 // Brick occupancy packed to one bit per brick (word w covers bricks 32w..32w+31).
@@ -223,7 +265,7 @@ extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, 
                                g);
         }
     } else if (phase == 2) {
-        if (ncell > 0) hipLaunchKernelGGL(k_sdf_finalize, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell, g, bricks);
+        if (ncell > 0) hipLaunchKernelGGL(k_sdf_finalize, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell, g);
     } else if (phase == 3) {
         if (bricks && g.brick_bits) {
             const int words = (nbrick + 31) / 32;
@@ -273,21 +315,10 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob*
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
     const SdfGrid g = j.g;
-    const uint32_t far = float_flip3(1e10f);
-    uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < j.ncell; i += gridDim.x * kSdfBlock) {
-        const uint32_t u = j.cells[i];
-        j.cells[i] = ifloat_flip3(u);
-        if (u != far) {
-            const int x = i % g.nx, r = i / g.nx, y = r % g.ny, z = r / g.ny;
-            const int x0 = max(x - kBrickHalo, 0) >> kBrickShift, x1 = min((x + kBrickHalo) >> kBrickShift, g.bx - 1);
-            const int y0 = max(y - kBrickHalo, 0) >> kBrickShift, y1 = min((y + kBrickHalo) >> kBrickShift, g.by - 1);
-            const int z0 = max(z - kBrickHalo, 0) >> kBrickShift, z1 = min((z + kBrickHalo) >> kBrickShift, g.bz - 1);
-            for (int bz = z0; bz <= z1; ++bz)
-                for (int by = y0; by <= y1; ++by)
-                    for (int bx = x0; bx <= x1; ++bx) bricks[(bz * g.by + by) * g.bx + bx] = 1;
-        }
-    }
+    const float inv_nx = 1.0f / (float)g.nx, inv_ny = 1.0f / (float)g.ny;
+    const bool fast = j.ncell <= (1 << 24);
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < j.ncell; i += gridDim.x * kSdfBlock)
+        finalize_cell(j.cells, i, g, inv_nx, inv_ny, fast);
 }
 
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_multi(const SdfJob* __restrict__ jobs) {
