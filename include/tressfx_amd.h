@@ -306,6 +306,36 @@ int tfx_sdf_download(tfx_sdf* sdf, void* stream, float* host_cells /* X*Y*Z */);
 int tfx_sdf_get_collision_params(const tfx_sdf* sdf, const tfx_hair* hair, tfx_sdf_collision_params* out);
 
 /* ------------------------------------------------------------------------ */
+/* Asset loading (TressFXAsset, src/TressFX/TressFXAsset.h:48-120)           */
+/* The reference's order: LoadHairData, GenerateFollowHairs, ProcessAsset,   */
+/* LoadBoneData; then tfx_asset_get_desc feeds tfx_hair_create.              */
+/* ------------------------------------------------------------------------ */
+typedef struct tfx_asset tfx_asset;
+/* LoadHairData, TressFXAsset.cpp:94-173: a .tfx file (version >= 4); guide
+ * strands padded to a multiple of 64 by copies of the last strand. */
+int tfx_asset_load_hair_data(const char* tfx_path, tfx_asset** out);
+/* GenerateFollowHairs, TressFXAsset.cpp:177-248; the offsets' random numbers
+ * come from a 32-bit LCG seeded with `seed` (the reference uses rand()). */
+int tfx_asset_generate_follow_hairs(tfx_asset* asset, int num_follow_hairs_per_guide, float tip_separation_factor,
+                                    float max_radius_around_guide_hair, unsigned int seed);
+/* ProcessAsset, TressFXAsset.cpp:250-399: tangents, thickness coefficients,
+ * rest lengths, triangle indices. */
+int tfx_asset_process(tfx_asset* asset);
+/* LoadBoneData, TressFXAsset.cpp:401-480: a .tfxbone file; its bone names are
+ * mapped to engine bone ids = their index in engine_bone_names (GetBoneIdByName). */
+int tfx_asset_load_bone_data(tfx_asset* asset, const char* tfxbone_path, const char* const* engine_bone_names,
+                             int num_engine_bones);
+/* The processed arrays (valid while the asset lives), for tfx_hair_create. */
+int tfx_asset_get_desc(const tfx_asset* asset, tfx_asset_desc* out);
+int tfx_asset_get_info(const tfx_asset* asset, int* file_strands, int* guide_strands, int* follow_per_guide,
+                       int* vertices_per_strand);
+/* Render-side arrays of ProcessAsset: strand UV float2[S], thickness float[N],
+ * triangle indices int32[6*S*(V-1)]. */
+int tfx_asset_get_render_data(const tfx_asset* asset, const float** strand_uv, const float** thickness_coeffs,
+                              const int32_t** triangle_indices, int* num_triangle_indices);
+int tfx_asset_destroy(tfx_asset* asset);
+
+/* ------------------------------------------------------------------------ */
 /* Collision-mesh skinning (TressFXBoneSkinning, src/TressFX/TressFXBoneSkinning.h:50-131) */
 /* ------------------------------------------------------------------------ */
 typedef struct tfx_mesh_skinning tfx_mesh_skinning;
@@ -323,6 +353,15 @@ typedef struct tfx_skinned_mesh_desc {
     int pad;
 } tfx_skinned_mesh_desc;
 /* Initialize, TressFXBoneSkinning.cpp:287-399 (device buffers, bounding cube). */
+typedef struct tfx_collision_mesh tfx_collision_mesh;
+/* LoadTressFXCollisionMeshData, TressFXBoneSkinning.cpp:100-286: a .tfxmesh
+ * text file; bone names mapped to engine ids (index in engine_bone_names, a
+ * name the skeleton lacks reads bone 0); follow_bone names the bone the
+ * bounding box follows. */
+int tfx_collision_mesh_load(const char* tfxmesh_path, const char* const* engine_bone_names, int num_engine_bones,
+                            const char* follow_bone, tfx_collision_mesh** out);
+int tfx_collision_mesh_get_desc(const tfx_collision_mesh* mesh, tfx_skinned_mesh_desc* out);
+int tfx_collision_mesh_destroy(tfx_collision_mesh* mesh);
 int tfx_mesh_skinning_create(const tfx_skinned_mesh_desc* mesh, tfx_mesh_skinning** out);
 int tfx_mesh_skinning_destroy(tfx_mesh_skinning* mesh);
 /* Update, TressFXBoneSkinning.cpp:460-504: new world-space bone matrices

@@ -281,6 +281,19 @@ EXPORTS = {
     "tfx_sdf_get_data_buffer": (_I, [_VP, ctypes.POINTER(_VP)]),
     "tfx_sdf_download": (_I, [_VP, _VP, c_float_p]),
     "tfx_sdf_get_collision_params": (_I, [_VP, _VP, ctypes.POINTER(SdfCollisionParams)]),
+    "tfx_asset_load_hair_data": (_I, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
+    "tfx_asset_generate_follow_hairs": (_I, [_VP, _I, _F, _F, ctypes.c_uint]),
+    "tfx_asset_process": (_I, [_VP]),
+    "tfx_asset_load_bone_data": (_I, [_VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), _I]),
+    "tfx_asset_get_desc": (_I, [_VP, ctypes.POINTER(AssetDesc)]),
+    "tfx_asset_get_info": (_I, [_VP] + [ctypes.POINTER(_I)] * 4),
+    "tfx_asset_get_render_data": (_I, [_VP, ctypes.POINTER(c_float_p), ctypes.POINTER(c_float_p),
+                                       ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)), ctypes.POINTER(_I)]),
+    "tfx_asset_destroy": (_I, [_VP]),
+    "tfx_collision_mesh_load": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), _I, ctypes.c_char_p,
+                                     ctypes.POINTER(_VP)]),
+    "tfx_collision_mesh_get_desc": (_I, [_VP, ctypes.POINTER(SkinnedMeshDesc)]),
+    "tfx_collision_mesh_destroy": (_I, [_VP]),
     "tfx_mesh_skinning_create": (_I, [ctypes.POINTER(SkinnedMeshDesc), ctypes.POINTER(_VP)]),
     "tfx_mesh_skinning_destroy": (_I, [_VP]),
     "tfx_mesh_skinning_update": (_I, [_VP, c_float_p, _I, _VP]),
@@ -857,3 +870,106 @@ class HairServer:
         h, sdf = ctypes.c_void_p(), ctypes.c_void_p()
         _check(self._lib.tfx_server_get_objects(self.handle, int(instance_id), ctypes.byref(h), ctypes.byref(sdf)))
         return _BorrowedHair(h.value)
+
+
+def _names(names):
+    names = [n.encode() if isinstance(n, str) else n for n in (names or [])]
+    arr = (ctypes.c_char_p * max(len(names), 1))(*names)
+    return arr, len(names)
+
+
+class TressFXAsset:
+    """TressFXAsset (src/TressFX/TressFXAsset.h:48-120) built by the library's
+    loaders, in the reference's order: LoadHairData, GenerateFollowHairs,
+    ProcessAsset, LoadBoneData; `hair_asset()` is what TressFXHairObject takes."""
+
+    def __init__(self):
+        self._lib = load_library()
+        self.handle = None
+
+    def close(self):
+        if self.handle:
+            self._lib.tfx_asset_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def LoadHairData(self, path):
+        self.close()
+        h = ctypes.c_void_p()
+        _check(self._lib.tfx_asset_load_hair_data(os.fsencode(path), ctypes.byref(h)))
+        self.handle = h
+        return True
+
+    def GenerateFollowHairs(self, num_follow_hairs_per_guide, tip_separation_factor, max_radius, seed=0):
+        _check(self._lib.tfx_asset_generate_follow_hairs(self.handle, int(num_follow_hairs_per_guide),
+                                                         ctypes.c_float(tip_separation_factor),
+                                                         ctypes.c_float(max_radius), ctypes.c_uint(seed)))
+        return True
+
+    def ProcessAsset(self):
+        _check(self._lib.tfx_asset_process(self.handle))
+        return True
+
+    def LoadBoneData(self, path, engine_bone_names):
+        arr, n = _names(engine_bone_names)
+        _check(self._lib.tfx_asset_load_bone_data(self.handle, os.fsencode(path), arr, n))
+        return True
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(self._lib.tfx_asset_get_info(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(file_strands=v[0].value, num_guide_strands=v[1].value, num_follow_strands_per_guide=v[2].value,
+                    num_vertices_per_strand=v[3].value)
+
+    def hair_asset(self):
+        """Copies of the processed arrays as a HairAsset."""
+        d = AssetDesc()
+        _check(self._lib.tfx_asset_get_desc(self.handle, ctypes.byref(d)))
+        G, F, V = d.num_guide_strands, d.num_follow_strands_per_guide, d.num_vertices_per_strand
+        S = G * (F + 1)
+        N = S * V
+
+        def arr(ptr, n):
+            return np.ctypeslib.as_array(ctypes.cast(ptr, c_float_p), shape=(n,)).copy()
+        return HairAsset(G, F, V, arr(d.positions, 4 * N).reshape(N, 4), arr(d.tangents, 4 * N).reshape(N, 4),
+                         arr(d.rest_lengths, N), arr(d.follow_root_offsets, 4 * S).reshape(S, 4),
+                         arr(d.bone_skinning_data, 8 * S).reshape(S, 8))
+
+    def render_data(self):
+        uv, th = c_float_p(), c_float_p()
+        tri, n = ctypes.POINTER(ctypes.c_int32)(), ctypes.c_int()
+        _check(self._lib.tfx_asset_get_render_data(self.handle, ctypes.byref(uv), ctypes.byref(th),
+                                                   ctypes.byref(tri), ctypes.byref(n)))
+        info = self.info()
+        S = info["num_guide_strands"] * (info["num_follow_strands_per_guide"] + 1)
+        N = S * info["num_vertices_per_strand"]
+        return (np.ctypeslib.as_array(uv, shape=(2 * S,)).reshape(S, 2).copy(),
+                np.ctypeslib.as_array(th, shape=(N,)).copy(),
+                np.ctypeslib.as_array(tri, shape=(n.value,)).copy())
+
+
+def load_collision_mesh(path, engine_bone_names, follow_bone=None):
+    """A .tfxmesh (TressFXBoneSkinning::LoadTressFXCollisionMeshData) as the
+    mesh dict TressFXBoneSkinning and HairServer take."""
+    lib = load_library()
+    arr, n = _names(engine_bone_names)
+    h = ctypes.c_void_p()
+    _check(lib.tfx_collision_mesh_load(os.fsencode(path), arr, n,
+                                       follow_bone.encode() if follow_bone else None, ctypes.byref(h)))
+    try:
+        d = SkinnedMeshDesc()
+        _check(lib.tfx_collision_mesh_get_desc(h, ctypes.byref(d)))
+        nv, nt = d.num_vertices, d.num_triangles
+        pos = np.ctypeslib.as_array(d.positions, shape=(3 * nv,)).reshape(nv, 3).copy()
+        nrm = np.ctypeslib.as_array(d.normals, shape=(3 * nv,)).reshape(nv, 3).copy()
+        skin = np.ctypeslib.as_array(ctypes.cast(d.skinning, c_float_p), shape=(8 * nv,)).reshape(nv, 8).copy()
+        idx = np.ctypeslib.as_array(ctypes.cast(d.indices, ctypes.POINTER(ctypes.c_uint32)),
+                                    shape=(3 * max(nt, 0),)).reshape(-1, 3).copy()
+        return dict(positions=pos, normals=nrm, skinning=skin, indices=idx, follow_bone=d.follow_bone)
+    finally:
+        lib.tfx_collision_mesh_destroy(h)

@@ -19,6 +19,7 @@ SOURCES = [
     "tfx_batch.hip",
     "tfx_mesh.hip",
     "tfx_server.hip",
+    "tfx_asset.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
