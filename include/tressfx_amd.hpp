@@ -14,6 +14,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "tressfx_amd.h"
@@ -38,8 +39,10 @@ struct TressFXSimulationSettings : tfx_simulation_settings {
     TressFXSimulationSettings() { tfx_simulation_settings_default(this); }
 };
 
-/* A processed TressFXAsset (TressFXAsset.h:50-100): host arrays, strand-major,
- * follow strands after their guide. */
+/* TressFXAsset (TressFXAsset.h:48-120): host arrays, strand-major, follow
+ * strands after their guide.  Filled either directly or by the loaders, in the
+ * reference's order LoadHairData, GenerateFollowHairs, ProcessAsset,
+ * LoadBoneData (the engine's bone names give the engine bone ids). */
 struct TressFXAsset {
     int m_numGuideStrands = 0;
     int m_numFollowStrandsPerGuide = 0;
@@ -50,8 +53,55 @@ struct TressFXAsset {
     std::vector<float> m_followRootOffsets;  /* float4 per strand */
     std::vector<tfx_bone_skinning_data> m_boneSkinningData;  /* per strand */
 
+    TressFXAsset() = default;
+    TressFXAsset(const TressFXAsset&) = delete;
+    TressFXAsset& operator=(const TressFXAsset&) = delete;
+    TressFXAsset(TressFXAsset&& o) noexcept { *this = std::move(o); }
+    TressFXAsset& operator=(TressFXAsset&& o) noexcept {
+        if (this != &o) {
+            m_numGuideStrands = o.m_numGuideStrands;
+            m_numFollowStrandsPerGuide = o.m_numFollowStrandsPerGuide;
+            m_numVerticesPerStrand = o.m_numVerticesPerStrand;
+            m_positions = std::move(o.m_positions);
+            m_tangents = std::move(o.m_tangents);
+            m_restLengths = std::move(o.m_restLengths);
+            m_followRootOffsets = std::move(o.m_followRootOffsets);
+            m_boneSkinningData = std::move(o.m_boneSkinningData);
+            tfx_asset_destroy(loader_);
+            loader_ = o.loader_;
+            o.loader_ = nullptr;
+        }
+        return *this;
+    }
+    ~TressFXAsset() { tfx_asset_destroy(loader_); }
+
     int NumTotalStrands() const { return m_numGuideStrands * (m_numFollowStrandsPerGuide + 1); }
     int NumTotalVertices() const { return NumTotalStrands() * m_numVerticesPerStrand; }
+
+    bool LoadHairData(const char* tfxPath) {
+        tfx_asset_destroy(loader_);
+        loader_ = nullptr;
+        check(tfx_asset_load_hair_data(tfxPath, &loader_));
+        return true;
+    }
+    bool GenerateFollowHairs(int numFollowHairsPerGuideHair, float tipSeparationFactor, float maxRadiusAroundGuideHair,
+                             unsigned int seed = 0) {
+        check(tfx_asset_generate_follow_hairs(loader_, numFollowHairsPerGuideHair, tipSeparationFactor,
+                                              maxRadiusAroundGuideHair, seed));
+        return true;
+    }
+    bool ProcessAsset() {
+        check(tfx_asset_process(loader_));
+        copy_from_loader();
+        return true;
+    }
+    bool LoadBoneData(const char* tfxbonePath, const std::vector<std::string>& engineBoneNames) {
+        std::vector<const char*> names;
+        for (const std::string& n : engineBoneNames) names.push_back(n.c_str());
+        check(tfx_asset_load_bone_data(loader_, tfxbonePath, names.data(), (int)names.size()));
+        copy_from_loader();
+        return true;
+    }
 
     tfx_asset_desc desc() const {
         tfx_asset_desc d{};
@@ -65,6 +115,22 @@ struct TressFXAsset {
         d.bone_skinning_data = m_boneSkinningData.empty() ? nullptr : m_boneSkinningData.data();
         return d;
     }
+
+private:
+    void copy_from_loader() {
+        tfx_asset_desc d{};
+        check(tfx_asset_get_desc(loader_, &d));
+        m_numGuideStrands = d.num_guide_strands;
+        m_numFollowStrandsPerGuide = d.num_follow_strands_per_guide;
+        m_numVerticesPerStrand = d.num_vertices_per_strand;
+        const size_t S = (size_t)NumTotalStrands(), N = (size_t)NumTotalVertices();
+        m_positions.assign(d.positions, d.positions + 4 * N);
+        m_tangents.assign(d.tangents, d.tangents + 4 * N);
+        m_restLengths.assign(d.rest_lengths, d.rest_lengths + N);
+        m_followRootOffsets.assign(d.follow_root_offsets, d.follow_root_offsets + 4 * S);
+        m_boneSkinningData.assign(d.bone_skinning_data, d.bone_skinning_data + S);
+    }
+    tfx_asset* loader_ = nullptr;
 };
 
 /* TressFXHairObject, TressFXHairObject.h:85-191 (simulation members). */
@@ -157,6 +223,60 @@ public:
         GetInitialBoundingBox(d.initial_bbox_min, d.initial_bbox_max);
         return d;
     }
+};
+
+/* TressFXBoneSkinning (TressFXBoneSkinning.h:50-131): a collision mesh loaded
+ * from a .tfxmesh and skinned on the GPU every Update; it is the mesh
+ * interface an SDF reads. */
+class TressFXBoneSkinning : public TressFXSDFInputMeshInterface {
+public:
+    TressFXBoneSkinning() = default;
+    ~TressFXBoneSkinning() override { tfx_mesh_skinning_destroy(m_); }
+    TressFXBoneSkinning(const TressFXBoneSkinning&) = delete;
+    TressFXBoneSkinning& operator=(const TressFXBoneSkinning&) = delete;
+
+    bool LoadTressFXCollisionMeshData(const char* tfxmeshPath, const std::vector<std::string>& engineBoneNames,
+                                      const char* followBone) {
+        std::vector<const char*> names;
+        for (const std::string& n : engineBoneNames) names.push_back(n.c_str());
+        tfx_collision_mesh* cm = nullptr;
+        check(tfx_collision_mesh_load(tfxmeshPath, names.data(), (int)names.size(), followBone, &cm));
+        tfx_skinned_mesh_desc d{};
+        int rc = tfx_collision_mesh_get_desc(cm, &d);
+        if (!rc) {
+            tfx_mesh_skinning_destroy(m_);
+            m_ = nullptr;
+            rc = tfx_mesh_skinning_create(&d, &m_);  /* Initialize: device buffers */
+        }
+        tfx_collision_mesh_destroy(cm);
+        check(rc);
+        return true;
+    }
+    /* Update(commandContext, time): the world-space bone matrices of the frame. */
+    void Update(void* stream, const float* boneMatrices, int numBones) {
+        check(tfx_mesh_skinning_update(m_, boneMatrices, numBones, stream));
+    }
+
+    const tfx_standard_vertex* GetMeshBuffer() override { return view().vertices; }
+    const unsigned int* GetTrimeshVertexIndicesBuffer() override { return view().indices; }
+    int GetNumMeshVertices() override { return view().num_vertices; }
+    int GetNumMeshTriangle() override { return view().num_triangles; }
+    void GetBoundingBox(float min[3], float max[3]) override {
+        const tfx_sdf_mesh_desc d = view();
+        for (int a = 0; a < 3; ++a) { min[a] = d.bbox_min[a]; max[a] = d.bbox_max[a]; }
+    }
+    void GetInitialBoundingBox(float min[3], float max[3]) override {
+        const tfx_sdf_mesh_desc d = view();
+        for (int a = 0; a < 3; ++a) { min[a] = d.initial_bbox_min[a]; max[a] = d.initial_bbox_max[a]; }
+    }
+
+private:
+    tfx_sdf_mesh_desc view() const {
+        tfx_sdf_mesh_desc d{};
+        check(tfx_mesh_skinning_get_mesh(m_, &d));
+        return d;
+    }
+    tfx_mesh_skinning* m_ = nullptr;
 };
 
 /* TressFXSDFCollisionSystem, TressFXSDFCollision.h:40-56. */

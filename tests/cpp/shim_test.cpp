@@ -2,13 +2,16 @@
 // reference's per-frame order (src/Simulation.cpp:50-131: SDF Update, then
 // Simulate and CollideWithHair) and, side by side, the same frames through the
 // plain C-ABI calls on a second set of objects; the two must end bit-identical.
-// Also checks the getters and that a bad asset raises tressfx_amd::Error.
+// Also checks the getters, that a bad asset raises tressfx_amd::Error, and
+// (given the fixture folder as argv[1]) the loaders and the skinned collision
+// mesh through the C++ classes.
 // Prints "shim_test OK" and exits 0 on success.
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "tressfx_amd.hpp"
@@ -123,7 +126,7 @@ void yaw_matrix(float yaw, float m[16]) {
 
 }  // namespace
 
-int main() {
+int main(int argc, char** argv) {
     try {
         TressFXSimulation::Initialize(0);
         const int G = 200, V = 16, F = 1;
@@ -208,6 +211,57 @@ int main() {
             threw = e.code() == TFX_ERR_INVALID_ARGUMENT;
         }
         REQUIRE(threw);
+        // The loaders and the skinned collision mesh through the C++ classes,
+        // beside the same calls through the C-ABI (argv[1]: the fixture folder).
+        if (argc > 1) {
+            const std::string dir = argv[1];
+            const std::string tfx = dir + "/ratboy_mohawk_70.tfx", bone = dir + "/ratboy_mohawk_70.tfxbone",
+                              msh = dir + "/ratboy_left_hand_400.tfxmesh";
+            std::vector<std::string> names = {"frenchHornMonster_head_JNT"};
+            TressFXAsset ra;
+            ra.LoadHairData(tfx.c_str());
+            ra.GenerateFollowHairs(1, 0.2f, 0.003f, 7u);
+            ra.ProcessAsset();
+            ra.LoadBoneData(bone.c_str(), names);
+            REQUIRE(ra.m_numGuideStrands == 128 && ra.m_numFollowStrandsPerGuide == 1 &&
+                    ra.m_numVerticesPerStrand == 32);
+            tfx_asset* ca = nullptr;
+            check(tfx_asset_load_hair_data(tfx.c_str(), &ca));
+            check(tfx_asset_generate_follow_hairs(ca, 1, 0.2f, 0.003f, 7u));
+            check(tfx_asset_process(ca));
+            const char* cnames[1] = {"frenchHornMonster_head_JNT"};
+            check(tfx_asset_load_bone_data(ca, bone.c_str(), cnames, 1));
+            tfx_asset_desc cd{};
+            check(tfx_asset_get_desc(ca, &cd));
+            REQUIRE(std::memcmp(cd.positions, ra.m_positions.data(), ra.m_positions.size() * 4) == 0);
+            REQUIRE(std::memcmp(cd.rest_lengths, ra.m_restLengths.data(), ra.m_restLengths.size() * 4) == 0);
+
+            std::vector<std::string> hand = {"frenchHornMonster_L_UpperArm_JNT", "frenchHornMonster_L_LowerArm_JNT",
+                                             "frenchHornMonster_L_Wrist_JNT"};
+            TressFXBoneSkinning skin;
+            skin.LoadTressFXCollisionMeshData(msh.c_str(), hand, hand[0].c_str());
+            REQUIRE(skin.GetNumMeshVertices() == 400 && skin.GetNumMeshTriangle() > 0);
+            TressFXSDFCollision sdf(&skin, 40, 1.0f);
+            TressFXHairObject rh(&ra);
+            TressFXSimulationSettings st;
+            std::vector<TressFXHairObject*> objs2{&rh};
+            float hb[16];
+            for (int frame = 0; frame < 3; ++frame) {
+                float mb[3 * 16];
+                for (int b = 0; b < 3; ++b) yaw_matrix(0.05f * frame * (b + 1), &mb[16 * b]);
+                skin.Update(nullptr, mb, 3);
+                sdf.Update(nullptr, system);
+                yaw_matrix(0.1f * frame, hb);
+                rh.UpdateBoneMatrices(hb, 1);
+                rh.UpdateSimulationParameters(&st, 1.0f / 60.0f);
+                sim.Simulate(nullptr, objs2);
+                sdf.CollideWithHair(nullptr, system, rh);
+            }
+            std::vector<float> rp((size_t)ra.NumTotalVertices() * 4);
+            check(tfx_hair_download_state(rh.handle(), nullptr, rp.data(), nullptr, nullptr, nullptr, nullptr));
+            for (float x : rp) REQUIRE(std::isfinite(x));
+            tfx_asset_destroy(ca);
+        }
     } catch (const Error& e) {
         std::fprintf(stderr, "shim_test: %s (code %d)\n", e.what(), e.code());
         return 1;
