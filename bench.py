@@ -41,11 +41,15 @@ def algorithmic_bytes_per_substep(G, F, V):
 
 
 def read_traffic(path):
+    """The PMC traffic record of the fused kernel, if it was measured on the
+    sources that are built now (else None)."""
+    from tressfx_amd.profile_summary import fused_source_digest
     try:
         with open(path) as f:
-            return json.load(f)
+            rec = json.load(f)
     except (OSError, ValueError):
         return None
+    return rec if rec.get("source_digest") == fused_source_digest() else None
 
 
 def cpu_baseline(wl, frames, nthreads, budget_s):
@@ -108,6 +112,89 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def sdf_rebuild_bytes(cells, mesh_vertices, triangles):
+    """Init write + finalize read/write of every cell (12 B), the mesh's
+    vertices (32 B) and indices (12 B per triangle)."""
+    return 12 * cells + 32 * mesh_vertices + 12 * triangles
+
+
+def fur_leg(torch, tfx, harness, stream, wl5, wls3, args):
+    """Config 5 alone -- the reference dispatch, the fused substep (pass
+    timers around every launch: the V = 8 fused kernel's duration) and the
+    batched form's graph-replayed frame over the same frames -- then, with
+    the config-4 mix, the fur as one heavy tenant stepped at 60 frames/s."""
+    import numpy as np
+    from tressfx_amd import synth
+    a = wl5.asset
+    G, F, V = a.num_guide_strands, a.num_follow_strands_per_guide, a.num_vertices_per_strand
+    n5, subs = a.num_total_vertices, wl5.substeps
+    cells = int(np.prod(wl5.sdf_grid[2]))
+    W5, K5, P5 = 2, args.fur_steps, args.fur_plain_steps
+    legs = {}
+    timers = None
+    for mode, k in (("graph", K5), ("fused", K5), ("plain", P5)):
+        if k <= 0:
+            continue
+        f5 = list(range(W5 + k))
+        d5 = harness.SkinnedFrameDriver(torch, wl5, f5, mode=mode)
+        with torch.cuda.stream(stream):
+            for f in range(W5):
+                d5.frame(f, stream.cuda_stream)
+            stream.synchronize()
+            if mode == "fused":
+                tfx.profiling_enable(True)
+                tfx.profiling_read("FusedSubstep")
+            el5, ms5 = _timed_frames(torch, stream, d5, f5[W5:])
+            if mode == "fused":
+                tfx.profiling_enable(False)
+                timers = tfx.profiling_read("FusedSubstep")
+        legs[mode] = {"value": n5 * subs * k / el5, "ms_per_step": el5 * 1e3 / k, "steps": k,
+                      "latency_ms": {"p50": float(np.percentile(ms5, 50)), "p99": float(np.percentile(ms5, 99))}}
+        _log(f"fur {mode} leg {el5 * 1e3 / k:.3f} ms/frame")
+        del d5
+    bytes_sub = algorithmic_bytes_per_substep(G, F, V)
+    bytes_frame = subs * bytes_sub + sdf_rebuild_bytes(cells, wl5.mesh.positions.shape[0], wl5.mesh.num_triangles)
+    out = {
+        "workload": f"config5: {G} guide strands x {V} vertices, {F} follow strands per guide ({n5} vertices), "
+                    f"roots on a {wl5.mesh.num_triangles}-triangle noisy ellipsoid (0.5 x 0.3 x 0.3 m) skinned "
+                    f"by 4 bones every frame, {wl5.sdf_grid[2][0]}^3 SDF rebuilt per frame, {subs} substeps, "
+                    f"length iterations 2, VSP 0.7",
+        "unit": "vertex-substeps/s",
+        "algorithmic_bytes_per_frame": bytes_frame,
+        "legs": legs,
+    }
+    if "graph" in legs:
+        g = legs["graph"]
+        out["value"] = g["value"]
+        out["frame_hbm_frac"] = bytes_frame / (g["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+        if "plain" in legs:
+            out["speedup_vs_plain"] = g["value"] / legs["plain"]["value"]
+    if timers and timers[1]:
+        avg_s = timers[0] / timers[1] / 1e3
+        out["roofline"] = {"bound": "hbm", "kernel": f"k_fused_substep<{V},*>", "avg_launch_us": avg_s * 1e6,
+                           "launches": timers[1], "algorithmic_bytes_per_launch": bytes_sub,
+                           "achieved": bytes_sub / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": bytes_sub / avg_s / 1e9 / HBM_PEAK_GBS}
+    if wls3 is not None and args.fur_served_seconds > 0:
+        srv = tfx.HairServer(hbm_budget_bytes=200 << 30, max_instances=1000, substeps_per_frame=2,
+                             substep_dt=float(np.float32(1.0 / synth.FRAME_RATE / 2)), max_inflight=2)
+        run = harness.ServedRun(srv, wls3, initial=args.served_initial, duration_s=args.fur_served_seconds,
+                                inst_rate=20.0, lifetime_s=30.0, req_rate=5000.0, ks=(1, 2, 4), tenants=[wl5],
+                                tenant_hz=60.0)
+        _log(f"served window with the fur tenant, {args.fur_served_seconds:.0f} s")
+        res = run.run(log=_log)
+        srv.close()
+        out["in_mix"] = {
+            "workload": "the config-4 request mix (as the served leg) with the fur resident as one more instance "
+                        "(3 substeps per frame), its client requesting one frame every 1/60 s",
+            "value": res["vertex_substeps"] / res["elapsed_s"], "unit": "vertex-substeps/s",
+            "latency_ms": {"p50": res["latency_p50_ms"], "p99": res["latency_p99_ms"], "max": res["latency_max_ms"]},
+            "tenant": res["tenants"][0], "passes": res["passes"],
+            "mean_instances_per_pass": res["mean_instances_per_pass"], "window_s": res["elapsed_s"],
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +207,11 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=12.0,
                     help="length of the served-form request window (0: skip)")
     ap.add_argument("--served-initial", type=int, default=256, help="instances resident when the window opens")
+    ap.add_argument("--fur-steps", type=int, default=10, help="timed frames of the config-5 fur workload")
+    ap.add_argument("--fur-plain-steps", type=int, default=3)
+    ap.add_argument("--fur-served-seconds", type=float, default=8.0,
+                    help="served window with the fur as a heavy tenant (0: skip)")
+    ap.add_argument("--no-fur", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=600)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -144,6 +236,10 @@ def main():
     if not args.no_batched:
         _log(f"generating the 256-instance workload with {host_workers} processes")
         wls3 = synth.config3(workers=host_workers)
+    wl5 = None
+    if not args.no_fur:
+        _log("generating the fur workload (1,000,000 x 8 on a 200,000-triangle body)")
+        wl5 = synth.config5()
 
     import torch
     import tressfx_amd as tfx
@@ -276,6 +372,10 @@ def main():
             "speedup_vs_plain": (sv / batched["plain"]["value"]) if batched and batched.get("plain") else None,
         }
 
+    fur = None
+    if wl5 is not None:
+        fur = fur_leg(torch, tfx, harness, stream, wl5, wls3, args)
+
     cpu = None
     if not args.no_cpu:
         cwl = synth.config2(num_guides=args.guides)
@@ -326,6 +426,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "batched": batched,
         "served": served,
+        "fur": fur,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

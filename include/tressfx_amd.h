@@ -438,7 +438,9 @@ typedef struct tfx_instance_desc {
     int sdf_dims[3];
     int sdf_num_cells_in_x;              /* ... else sized from the mesh (TressFXSDFCollision.cpp:38-86) */
     float collision_margin;              /* in cells */
-    int pad;
+    int substeps;                        /* substeps per frame of this instance, each frame/substeps long
+                                            (frame = server substeps_per_frame x substep_dt); 0 = the
+                                            server's substeps_per_frame */
     tfx_simulation_settings settings;
     const float* capsules0;              /* float4[num_capsules] (or NULL) */
     const float* capsules1;
@@ -470,9 +472,10 @@ int tfx_server_admit(tfx_server* server, const tfx_instance_desc* desc, int* ins
 int tfx_server_retire(tfx_server* server, int instance_id);
 /* Queues `frames` frames for an instance, run in submission order:
  * hair_bones[frames][substeps][num_bones*16] (the hair object's bone
- * matrices for each substep), mesh_bones[frames][num_mesh_bones*16] (the
- * collision mesh's, per frame; NULL keeps the current ones),
- * wind[frames][substeps] (wind magnitude overrides, or NULL).  ticket NULL:
+ * matrices for each of the instance's substeps),
+ * mesh_bones[frames][num_mesh_bones*16] (the collision mesh's, per frame;
+ * NULL keeps the current ones), wind[frames][substeps] (wind magnitude
+ * overrides, or NULL).  ticket NULL:
  * nothing to poll; the request only counts in the statistics. */
 int tfx_server_submit(tfx_server* server, int instance_id, int frames, const float* hair_bones, int num_bones,
                       const float* mesh_bones, int num_mesh_bones, const float* wind, unsigned long long* ticket);

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import tressfx_amd as tfx
-from tressfx_amd import synth
+from tressfx_amd import harness, synth
 
 from tfx_testutil import first_mismatch
 
@@ -125,6 +125,65 @@ def test_alone_in_mix_and_per_object_are_identical(gpu):
     srv.close()
 
 
+def test_own_substeps_and_four_bone_body_in_a_mix(gpu):
+    """A reduced config-5 fur instance (3 substeps in a server whose frame is 2
+    substeps; a body skinned by 4 bones, the hair by the same 4) gives the same
+    result alone, in a mix with 2-substep instances, and through the per-object
+    dispatch at dt = frame / 3; the 2-substep instances it ran beside match
+    their own per-object dispatch."""
+    print("[server] own substeps + 4-bone body", flush=True)
+    fur = synth.config5(num_guides=3000, freq=20, grid_n=64, cell=0.01)
+    fmesh = fur.mesh.skinned_mesh()
+    frames = 4
+    hb, mb, _ = harness.frame_inputs(fur, frames)
+    assert hb.shape == (frames, 3, 4, 16) and mb.shape == (frames, 4, 16)
+    dt3 = np.float32(np.float64(DT) * SUBSTEPS / 3)
+
+    def admit_fur(srv):
+        return srv.admit(fur.asset, fur.settings, mesh=fmesh, sdf_grid=fur.sdf_grid,
+                         collision_margin=fur.collision_margin, substeps=3)
+
+    srv = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
+    fid = admit_fur(srv)
+    srv.wait(srv.submit(fid, hb, mb))
+    alone = srv.hair(fid).download_state()
+    srv.close()
+
+    others = [make_instance(20 + k, G, V) for k, (G, V) in enumerate([(800, 16), (300, 32), (200, 64)])]
+    srv = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT), max_inflight=2)
+    ids = [admit(srv, wl, mesh) for wl, mesh in others]
+    fid = admit_fur(srv)
+    tickets = [srv.submit(i, *inputs(wl, 0, 3)) for i, (wl, _) in zip(ids, others)]
+    tickets.append(srv.submit(fid, hb[:2], mb[:2]))
+    tickets += [srv.submit(i, *inputs(wl, 3, 2)) for i, (wl, _) in zip(ids, others)]
+    tickets.append(srv.submit(fid, hb[2:], mb[2:]))
+    for t in tickets:
+        srv.wait(t)
+    st = srv.stats()
+    assert st["mean_instances_per_pass"] > 1.5, "requests were not batched"
+    compare(srv.hair(fid).download_state(), alone, "fur in the mix vs alone")
+    compare(srv.hair(ids[0]).download_state(), per_object(*others[0], 5), "2-substep neighbour vs per-object")
+    compare(srv.hair(ids[2]).download_state(), per_object(*others[2], 5), "V=64 neighbour vs per-object")
+    srv.close()
+
+    hair = tfx.TressFXHairObject(fur.asset)
+    sk = tfx.TressFXBoneSkinning(fmesh["positions"], fmesh["skinning"], fmesh["indices"], normals=fmesh["normals"])
+    origin, cell, dims = fur.sdf_grid
+    sdf = tfx.TressFXSDFCollision(sk.mesh(), collision_margin=fur.collision_margin, explicit=(origin, cell, dims))
+    system = tfx.TressFXSDFCollisionSystem()
+    sim = tfx.TressFXSimulation()
+    for i in range(frames):
+        sk.Update(mb[i])
+        sdf.SetMesh(sk.mesh())
+        sdf.Update(system)
+        for s in range(3):
+            hair.UpdateBoneMatrices(hb[i, s])
+            hair.UpdateSimulationParameters(fur.settings, dt3)
+            sim.Simulate([hair])
+            sdf.CollideWithHair(system, hair)
+    compare(alone, hair.download_state(), "fur served vs per-object dispatch at dt = frame / 3")
+
+
 def test_admission_budget_and_retire(gpu):
     print("[server] admission", flush=True)
     wl, mesh = make_instance(7, 500, 16)
@@ -165,4 +224,9 @@ def test_server_limits_and_errors(gpu):
     hb, mb, wind = inputs(wl, 0, 1)
     with pytest.raises(tfx.TFXError):
         srv.submit(a + 100, hb, mb, wind)
+    srv.close()
+    srv = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
+    with pytest.raises(tfx.TFXError) as e:
+        srv.admit(wl.asset, wl.settings, mesh=mesh, sdf_grid=wl.sdf_grid, substeps=17)
+    assert e.value.code == 1
     srv.close()

@@ -213,7 +213,7 @@ class InstanceDesc(ctypes.Structure):
         ("sdf_dims", ctypes.c_int * 3),
         ("sdf_num_cells_in_x", ctypes.c_int),
         ("collision_margin", ctypes.c_float),
-        ("pad", ctypes.c_int),
+        ("substeps", ctypes.c_int),
         ("settings", SimulationSettings),
         ("capsules0", c_float_p),
         ("capsules1", c_float_p),
@@ -769,6 +769,7 @@ class HairServer:
         _check(self._lib.tfx_server_create(ctypes.byref(c), ctypes.byref(h)))
         self.handle = h
         self.substeps = int(substeps_per_frame)
+        self._substeps = {}    # instance id -> its substeps per frame
 
     def close(self):
         if self.handle:
@@ -782,11 +783,13 @@ class HairServer:
             pass
 
     def admit(self, asset, settings, mesh=None, sdf_grid=None, sdf_num_cells_in_x=0, collision_margin=0.5,
-              capsules=None):
+              capsules=None, substeps=0):
         """mesh: dict(positions, skinning, indices, normals=None, follow_bone=0);
-        sdf_grid: (origin, cell, dims) explicit grid, else sdf_num_cells_in_x.
+        sdf_grid: (origin, cell, dims) explicit grid, else sdf_num_cells_in_x;
+        substeps: the instance's substeps per frame (0: the server's).
         Raises TFXError (code TFX_ERR_ADMISSION = 4) when it does not fit now."""
         d = InstanceDesc()
+        d.substeps = int(substeps)
         ad = asset.desc()
         d.asset = ctypes.pointer(ad)
         keep = [ad, asset]
@@ -822,23 +825,27 @@ class HairServer:
             keep += [c0, c1]
         out = ctypes.c_int()
         _check(self._lib.tfx_server_admit(self.handle, ctypes.byref(d), ctypes.byref(out)))
+        self._substeps[out.value] = int(substeps) or self.substeps
         return out.value
 
     def retire(self, instance_id):
         _check(self._lib.tfx_server_retire(self.handle, int(instance_id)))
+        self._substeps.pop(int(instance_id), None)
 
     def submit(self, instance_id, hair_bones, mesh_bones=None, wind=None, track=True):
-        """hair_bones float32[frames, substeps, nb, 16] (or [frames, substeps, 16]);
-        mesh_bones float32[frames, nmb, 16] or None; wind float32[frames, substeps] or None."""
+        """hair_bones float32[frames, substeps, nb, 16] (or [frames, substeps, 16]) with the
+        instance's substeps; mesh_bones float32[frames, nmb, 16] or None;
+        wind float32[frames, substeps] or None."""
+        subs = self._substeps.get(int(instance_id), self.substeps)
         hb = np.ascontiguousarray(hair_bones, np.float32)
         frames = hb.shape[0]
-        hb = hb.reshape(frames, self.substeps, -1)
+        hb = hb.reshape(frames, subs, -1)
         nb = hb.shape[2] // 16
         mb, nmb = None, 0
         if mesh_bones is not None:
             mb = np.ascontiguousarray(mesh_bones, np.float32).reshape(frames, -1)
             nmb = mb.shape[1] // 16
-        w = None if wind is None else np.ascontiguousarray(wind, np.float32).reshape(frames, self.substeps)
+        w = None if wind is None else np.ascontiguousarray(wind, np.float32).reshape(frames, subs)
         t = ctypes.c_ulonglong()
         _check(self._lib.tfx_server_submit(self.handle, int(instance_id), frames, _fptr(hb), nb,
                                            _fptr(mb) if mb is not None else None, nmb,

@@ -31,9 +31,9 @@ struct Request {
     int issued = 0;  // frames staged into passes
     int done = 0;    // frames completed on the GPU
     int nb = 0, nmb = 0;
-    std::vector<float> hair_bones;  // [frames][substeps][nb * 16]
+    std::vector<float> hair_bones;  // [frames][instance substeps][nb * 16]
     std::vector<float> mesh_bones;  // [frames][nmb * 16] (empty: keep)
-    std::vector<float> wind;        // [frames][substeps] (empty: settings)
+    std::vector<float> wind;        // [frames][instance substeps] (empty: settings)
     double t_submit = 0.0;
     bool keep_result = true;  // false: counted in the statistics only
 };
@@ -44,6 +44,8 @@ struct Instance {
     tfx_sdf* sdf = nullptr;
     tfx_mesh_skinning* mesh = nullptr;
     tfx_simulation_settings settings{};
+    int substeps = 1;     // substeps per frame
+    float dt = 0.0f;      // substep length
     long long bytes = 0;
     std::deque<Request*> queue;  // unfinished requests, oldest first
     int inflight = 0;            // frames staged but not completed
@@ -222,7 +224,9 @@ int ensure_slot(Slot& sl, size_t bytes) {
 // and a request's inputs are immutable after submission.
 int launch_pass(tfx_server* s, Pass& pass) {
     const int n = (int)pass.frames.size();
-    const int subs = s->cfg.substeps_per_frame;
+    // Substep `sub` advances the members with more than `sub` substeps.
+    int subs = 1;
+    for (auto& fr : pass.frames) subs = std::max(subs, fr.second->substeps);
     // ---- layout
     std::vector<int> mesh_members, sdf_members;
     size_t mesh_bone_floats = 0, hair_bone_floats = 0;
@@ -246,24 +250,31 @@ int launch_pass(tfx_server* s, Pass& pass) {
     const size_t sub_bytes = sub_params + align256(hair_bone_floats * sizeof(float));
     const size_t off_tables = off_sub + (size_t)subs * sub_bytes;
     const size_t off_items = align256(off_tables + (size_t)subs * n * sizeof(DevHair));
-    // work items per strand length (identical for every substep)
+    // work items per strand length, members with more substeps first, so
+    // that substep `sub` runs a prefix of each strand length's items
     std::vector<tfx::WorkItem> items;
-    int vbegin[65] = {0}, vcount[65] = {0};
+    int vbegin[65] = {0};
+    std::vector<int> vcount((size_t)65 * subs, 0);  // [V][sub]
     Shape shapes[65];
     std::vector<tfx_hair*> hairs(n);
     for (int m = 0; m < n; ++m) hairs[m] = pass.frames[m].second->hair;
     const int cus = cu_count(s->cfg.device);
     for (int V = 4; V <= 64; V *= 2) {
         vbegin[V] = (int)items.size();
-        bool any = false;
-        for (int m = 0; m < n; ++m) any = any || hairs[m]->V == V;
-        if (!any) continue;
+        std::vector<int> mem;
+        for (int m = 0; m < n; ++m)
+            if (hairs[m]->V == V) mem.push_back(m);
+        if (mem.empty()) continue;
+        std::stable_sort(mem.begin(), mem.end(), [&](int a, int b) {
+            return pass.frames[a].second->substeps > pass.frames[b].second->substeps;
+        });
         shapes[V] = choose_shape(&s->shape, V, hairs.data(), n, cus);
         const int B = tfx_fused_block_vertices(V, shapes[V].variant) / V;
-        for (int m = 0; m < n; ++m)
-            if (hairs[m]->V == V)
-                for (int g0 = 0; g0 < hairs[m]->G; g0 += B) items.push_back(tfx::WorkItem{m, g0});
-        vcount[V] = (int)items.size() - vbegin[V];
+        for (int m : mem) {
+            for (int g0 = 0; g0 < hairs[m]->G; g0 += B) items.push_back(tfx::WorkItem{m, g0});
+            const int upto = (int)items.size() - vbegin[V];
+            for (int sub = 0; sub < pass.frames[m].second->substeps; ++sub) vcount[(size_t)V * subs + sub] = upto;
+        }
     }
     const size_t total = off_items + items.size() * sizeof(tfx::WorkItem);
     // ---- slot
@@ -331,15 +342,17 @@ int launch_pass(tfx_server* s, Pass& pass) {
         for (int m = 0; m < n; ++m) {
             Request* r = pass.frames[m].first;
             Instance* in = pass.frames[m].second;
+            if (sub >= in->substeps) continue;  // no work item reads this member's parameters
             tfx_hair* h = in->hair;
+            const size_t at = (size_t)r->issued * in->substeps + sub;
             if (r->nb > 0) {
-                const size_t off = ((size_t)r->issued * subs + sub) * (size_t)r->nb * 16;
-                rc = tfx_hair_update_bone_matrices(h, &r->hair_bones[off], std::min(r->nb, TFX_MAX_NUM_BONES));
+                rc = tfx_hair_update_bone_matrices(h, &r->hair_bones[at * (size_t)r->nb * 16],
+                                                   std::min(r->nb, TFX_MAX_NUM_BONES));
                 if (rc) return rc;
             }
             tfx_simulation_settings st = in->settings;
-            if (!r->wind.empty()) st.wind_magnitude = r->wind[(size_t)r->issued * subs + sub];
-            rc = tfx_hair_update_simulation_parameters(h, &st, s->cfg.substep_dt);
+            if (!r->wind.empty()) st.wind_magnitude = r->wind[at];
+            rc = tfx_hair_update_simulation_parameters(h, &st, in->dt);
             if (rc) return rc;
             pack_dev_params(h, in->sdf, params[m]);
             std::memcpy(bone_block + hair_bone_off[m], h->bones_h.data(), (size_t)16 * h->bones_used * sizeof(float));
@@ -360,7 +373,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
     for (int m = 0; m < n; ++m) {
         Request* r = pass.frames[m].first;
         tfx_hair* h = pass.frames[m].second->hair;
-        pass.vertex_substeps += (long long)h->N * subs;
+        pass.vertex_substeps += (long long)h->N * pass.frames[m].second->substeps;
         r->issued += 1;
     }
     // ---- launches
@@ -378,10 +391,11 @@ int launch_pass(tfx_server* s, Pass& pass) {
     const tfx::WorkItem* ditems = reinterpret_cast<const tfx::WorkItem*>(D + off_items);
     for (int sub = 0; sub < subs; ++sub)
         for (int V = 4; V <= 64; V *= 2) {
-            if (!vcount[V]) continue;
-            const int grid = std::min(vcount[V], shapes[V].resident);
-            HIP_TRY(tfx_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], vcount[V],
-                                     grid, st));
+            const int cnt = vcount[(size_t)V * subs + sub];
+            if (!cnt) continue;
+            const int grid = std::min(cnt, shapes[V].resident);
+            HIP_TRY(tfx_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], cnt, grid,
+                                     st));
         }
     HIP_TRY(hipEventRecord(pass.ev, st));
     if (!sl.ev) HIP_TRY(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
@@ -505,6 +519,7 @@ extern "C" int tfx_server_destroy(tfx_server* s) {
 
 extern "C" int tfx_server_admit(tfx_server* s, const tfx_instance_desc* d, int* id) {
     CHECK_ARG(s && d && d->asset && id, "tfx_server_admit: null argument");
+    CHECK_ARG(d->substeps >= 0 && d->substeps <= 16, "tfx_server_admit: substeps must be in [0, 16]");
     long long bytes = hair_bytes(d->asset);
     long long cells = 0;
     if (d->mesh) {
@@ -534,6 +549,11 @@ extern "C" int tfx_server_admit(tfx_server* s, const tfx_instance_desc* d, int* 
     Instance* in = new Instance();
     in->settings = d->settings;
     in->bytes = bytes;
+    // a frame lasts substeps_per_frame x substep_dt whatever the instance's substeps
+    in->substeps = d->substeps > 0 ? d->substeps : s->cfg.substeps_per_frame;
+    in->dt = in->substeps == s->cfg.substeps_per_frame
+                 ? s->cfg.substep_dt
+                 : (float)((double)s->cfg.substep_dt * s->cfg.substeps_per_frame / in->substeps);
     int rc = tfx_hair_create(d->asset, &in->hair);
     if (!rc && d->num_capsules > 0)
         rc = tfx_hair_set_collision_capsules(in->hair, d->capsules0, d->capsules1, d->num_capsules);
@@ -588,25 +608,21 @@ extern "C" int tfx_server_submit(tfx_server* s, int id, int frames, const float*
     CHECK_ARG(num_bones >= 0 && num_mesh_bones >= 0 && (hair_bones || num_bones == 0) &&
                   (mesh_bones || num_mesh_bones == 0),
               "tfx_server_submit: bad bone arguments");
-    const int subs = s->cfg.substeps_per_frame;
-    Request* r = new Request();
-    r->frames = frames;
-    r->nb = num_bones;
-    r->nmb = mesh_bones ? num_mesh_bones : 0;
-    if (num_bones) r->hair_bones.assign(hair_bones, hair_bones + (size_t)frames * subs * num_bones * 16);
-    if (r->nmb) r->mesh_bones.assign(mesh_bones, mesh_bones + (size_t)frames * r->nmb * 16);
-    if (wind) r->wind.assign(wind, wind + (size_t)frames * subs);
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->error) {
-            delete r;
-            return fail(s->error, "tfx_server: %s", s->error_msg.c_str());
-        }
+        if (s->error) return fail(s->error, "tfx_server: %s", s->error_msg.c_str());
         auto it = s->instances.find(id);
-        if (it == s->instances.end() || it->second->retiring) {
-            delete r;
+        if (it == s->instances.end() || it->second->retiring)
             return fail(TFX_ERR_NOT_FOUND, "tfx_server_submit: no instance %d", id);
-        }
+        // the inputs are sized by the instance's substeps (fixed at admission)
+        const int subs = it->second->substeps;
+        Request* r = new Request();
+        r->frames = frames;
+        r->nb = num_bones;
+        r->nmb = mesh_bones ? num_mesh_bones : 0;
+        if (num_bones) r->hair_bones.assign(hair_bones, hair_bones + (size_t)frames * subs * num_bones * 16);
+        if (r->nmb) r->mesh_bones.assign(mesh_bones, mesh_bones + (size_t)frames * r->nmb * 16);
+        if (wind) r->wind.assign(wind, wind + (size_t)frames * subs);
         r->ticket = s->next_ticket++;
         r->t_submit = now_s();
         r->keep_result = ticket != nullptr;

@@ -223,12 +223,93 @@ class PlainMultiDriver:
 
 
 def served_mesh(wl):
-    """The collision mesh of a workload as the served form takes it: the rest
-    mesh (static noise) skinned every frame by the head bone (bone 0)."""
+    """The collision mesh of a workload as the served form takes it: a body
+    with its own skinning (FurBody) as it is, else the rest mesh (static
+    noise) skinned every frame by the head bone (bone 0)."""
+    if hasattr(wl.mesh, "skinned_mesh"):
+        return wl.mesh.skinned_mesh()
     verts = wl.mesh.vertices(0.0)
     skin = np.zeros((verts.shape[0], 8), np.float32)
     skin[:, 4] = 1.0
     return dict(positions=verts[:, :3], normals=verts[:, 4:7], skinning=skin, indices=wl.mesh.indices, follow_bone=0)
+
+
+def frame_inputs(wl, horizon):
+    """Per-frame inputs of a workload for frames 0..horizon-1: hair bones at
+    the end of each of its substeps [H, substeps, (nb,) 16], the collision
+    mesh's bones at the frame's start [H, (nb,) 16] and wind [H, substeps]."""
+    sub = wl.substeps
+    f = np.arange(horizon, dtype=np.float64)
+    t_sub = (f[:, None] + (np.arange(sub)[None, :] + 1.0) / sub) / synth.FRAME_RATE
+    hb = synth.trajectory_bones(wl.bone_fn, t_sub)
+    mb = synth.trajectory_bones(wl.bone_fn, f / synth.FRAME_RATE)
+    wind = (np.vectorize(wl.wind_fn)(t_sub) if wl.wind_fn is not None
+            else np.full(t_sub.shape, wl.settings.wind_magnitude)).astype(np.float32)
+    return np.ascontiguousarray(hb), np.ascontiguousarray(mb), np.ascontiguousarray(wind)
+
+
+class SkinnedFrameDriver:
+    """One workload whose collider (a FurBody) is skinned on the GPU every
+    frame -- TressFXBoneSkinning::Update, then the SDF Update -- stepped with
+    the reference dispatch (`mode="plain"`: per substep Simulate +
+    CollideWithHair), the fused substep (`"fused"`) or the batched form with
+    the frame as a replayed hipGraph (`"graph"`).  Inputs for every frame of
+    `frames` are evaluated beforehand."""
+
+    def __init__(self, torch, wl, frames, mode="graph"):
+        from . import HairBatch, TressFXBoneSkinning
+        assert mode in ("plain", "fused", "graph")
+        self.wl, self.mode = wl, mode
+        self.frames = list(frames)
+        self.first = self.frames[0]
+        m = wl.mesh.skinned_mesh()
+        self.skin = TressFXBoneSkinning(m["positions"], m["skinning"], m["indices"], normals=m["normals"],
+                                        follow_bone=m["follow_bone"])
+        origin, cell, dims = wl.sdf_grid
+        self.sdf = TressFXSDFCollision(self.skin.mesh(), collision_margin=wl.collision_margin,
+                                       explicit=(origin, cell, dims))
+        self.hair = TressFXHairObject(wl.asset)
+        self.sim = TressFXSimulation()
+        self.system = TressFXSDFCollisionSystem()
+        self.substeps = wl.substeps
+        self.dt_sub = np.float32(1.0 / synth.FRAME_RATE / self.substeps)
+        hb, mb, wind = frame_inputs(wl, self.frames[-1] + 1)
+        self.hb = hb[self.first:].reshape(len(self.frames), self.substeps, -1)
+        self.mb = mb[self.first:].reshape(len(self.frames), -1, 16)
+        self.wind = wind[self.first:]
+        self.batch = None
+        if mode == "graph":
+            self.batch = HairBatch()
+            self.batch.add(self.hair, self.sdf)
+
+    @property
+    def vertices(self):
+        return self.wl.asset.num_total_vertices
+
+    def frame(self, f, stream=None):
+        k = f - self.first
+        wl = self.wl
+        self.skin.Update(self.mb[k], stream)
+        if self.batch is not None:
+            self.sdf.SetMesh(self.skin.mesh())
+            self.batch.begin_frame()
+            for s in range(self.substeps):
+                self.batch.stage_substep(self.hb[k, s][None], wl.settings, self.dt_sub,
+                                         self.wind[k, s:s + 1] if wl.wind_fn is not None else None)
+            self.batch.run_frame(stream, use_graph=True)
+            return
+        self.sdf.SetMesh(self.skin.mesh())
+        self.sdf.Update(self.system, stream)
+        for s in range(self.substeps):
+            self.hair.UpdateBoneMatrices(self.hb[k, s].reshape(-1, 16))
+            if wl.wind_fn is not None:
+                wl.settings.wind_magnitude = float(self.wind[k, s])
+            self.hair.UpdateSimulationParameters(wl.settings, self.dt_sub)
+            if self.mode == "fused":
+                self.sim.SimulateFused([self.hair], [self.sdf], stream)
+            else:
+                self.sim.Simulate([self.hair], stream)
+                self.sdf.CollideWithHair(self.system, self.hair, stream)
 
 
 class ServedRun:
@@ -237,13 +318,19 @@ class ServedRun:
     exponential lifetimes, and step requests of k frames (k uniform over `ks`)
     arriving as a Poisson process on random resident instances.  A refused
     admission waits in a FIFO and is retried as memory frees up.  Per-frame
-    inputs (bones for every substep, the collision mesh's bone, wind) are
-    evaluated for `horizon` frames per workload before the run."""
+    inputs (bones for every substep, the collision mesh's bones, wind) are
+    evaluated for `horizon` frames per workload before the run.
+
+    `tenants`: workloads resident for the whole run outside the arrival
+    process, each driven by its own client at `tenant_hz` requests of one
+    frame per second; their request latencies are reported separately."""
 
     def __init__(self, server, workloads, initial, duration_s, inst_rate=20.0, lifetime_s=30.0, req_rate=5000.0,
-                 ks=(1, 2, 4), seed=4, horizon=512):
+                 ks=(1, 2, 4), seed=4, horizon=512, tenants=(), tenant_hz=60.0):
         self.server = server
         self.wls = list(workloads)
+        self.tenants = list(tenants)
+        self.tenant_hz = float(tenant_hz)
         self.initial = int(initial)
         self.duration = float(duration_s)
         self.ks = tuple(ks)
@@ -260,27 +347,23 @@ class ServedRun:
         self.req_k = rng.choice(self.ks, n_req)
         self.lifetime_s = float(lifetime_s)
         # per-workload inputs for `horizon` frames
-        sub = self.substeps
-        self.inputs = []
-        for wl in self.wls:
-            f = np.arange(self.horizon, dtype=np.float64)
-            t_sub = (f[:, None] + (np.arange(sub)[None, :] + 1.0) / sub) / synth.FRAME_RATE
-            hb = synth.trajectory_bones(wl.bone_fn, t_sub)                # [H, sub, 16]
-            mb = synth.trajectory_bones(wl.bone_fn, f / synth.FRAME_RATE)  # [H, 16]
-            wind = (np.vectorize(wl.wind_fn)(t_sub) if wl.wind_fn is not None
-                    else np.full(t_sub.shape, wl.settings.wind_magnitude)).astype(np.float32)
-            self.inputs.append((np.ascontiguousarray(hb), np.ascontiguousarray(mb), np.ascontiguousarray(wind)))
+        self.inputs = [frame_inputs(wl, self.horizon) for wl in self.wls]
         self.meshes = [served_mesh(wl) for wl in self.wls]
+        self.tenant_inputs = [frame_inputs(wl, self.horizon) for wl in self.tenants]
+        self.tenant_meshes = [served_mesh(wl) for wl in self.tenants]
 
-    def _admit(self, w):
-        wl = self.wls[w]
+    def _admit_wl(self, wl, mesh):
         try:
-            return self.server.admit(wl.asset, wl.settings, mesh=self.meshes[w], sdf_grid=wl.sdf_grid,
-                                     collision_margin=wl.collision_margin, capsules=wl.capsules)
+            return self.server.admit(wl.asset, wl.settings, mesh=mesh, sdf_grid=wl.sdf_grid,
+                                     collision_margin=wl.collision_margin, capsules=wl.capsules,
+                                     substeps=wl.substeps if wl.substeps != self.substeps else 0)
         except Exception as e:  # noqa: BLE001 - admission refusals come back as TFXError code 4
             if getattr(e, "code", None) == 4:
                 return None
             raise
+
+    def _admit(self, w):
+        return self._admit_wl(self.wls[w], self.meshes[w])
 
     def run(self, log=None):
         import time
@@ -289,6 +372,16 @@ class ServedRun:
         info = {}              # id -> [workload, next frame, departure time]
         refused = 0
         pending = []           # (workload) waiting for admission
+        tenant_ids = []
+        for wl, mesh in zip(self.tenants, self.tenant_meshes):
+            iid = self._admit_wl(wl, mesh)
+            if iid is None:
+                raise RuntimeError("a tenant does not fit the admission budget")
+            tenant_ids.append(iid)
+        tenant_next = [0] * len(tenant_ids)     # next frame
+        tenant_tickets = [[] for _ in tenant_ids]
+        tenant_period = 1.0 / self.tenant_hz if self.tenant_hz > 0 else np.inf
+        next_tenant = 0.0
         for k in range(self.initial):
             w = k % len(self.wls)
             iid = self._admit(w)
@@ -309,8 +402,15 @@ class ServedRun:
             ta = self.arrivals[ia] if ia < len(self.arrivals) else np.inf
             tr = self.requests[ir] if ir < len(self.requests) else np.inf
             due = min(ta, tr)
+            if tenant_ids and now >= next_tenant:
+                for j, iid in enumerate(tenant_ids):
+                    hb, mb, wind = self.tenant_inputs[j]
+                    f = tenant_next[j] % self.horizon
+                    tenant_tickets[j].append(srv.submit(iid, hb[f:f + 1], mb[f:f + 1], wind[f:f + 1]))
+                    tenant_next[j] += 1
+                next_tenant += tenant_period
             if due > now:
-                time.sleep(min(due - now, 0.002))
+                time.sleep(max(0.0, min(due - now, next_tenant - now if tenant_ids else 1.0, 0.002)))
                 continue
             # departures
             if resident and now >= next_retry:
@@ -357,8 +457,20 @@ class ServedRun:
         srv.drain()
         t1 = time.perf_counter() - t0
         st = srv.stats()
-        return {
+        out = {
             "window_s": t1, "submit_window_s": t_sub, "requests_submitted": submitted, "frames_submitted": frames,
             "offered_request_rate": submitted / t_sub if t_sub > 0 else 0.0, "instances_retired": retired,
             "admissions_refused": refused, "resident_at_end": len(resident), **st,
         }
+        if tenant_ids:
+            tl = []
+            for j, iid in enumerate(tenant_ids):
+                lat = np.array([srv.wait(t) for t in tenant_tickets[j]])
+                wl = self.tenants[j]
+                tl.append({"frames": int(tenant_next[j]), "vertices": int(wl.asset.num_total_vertices),
+                           "substeps": int(wl.substeps),
+                           "vertex_substeps_per_s": tenant_next[j] * wl.asset.num_total_vertices * wl.substeps / t1,
+                           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                          "max": float(lat.max())}})
+            out["tenants"] = tl
+        return out

@@ -4,16 +4,33 @@ FETCH_SIZE and WRITE_SIZE, calibrated against a device copy of known size.
 
     python -m tressfx_amd.profile_summary stats  KERNEL_STATS_CSV KERNEL_SUBSTRING
     python -m tressfx_amd.profile_summary traffic FETCH_CSV WRITE_CSV KERNEL_SUBSTRING COPY_BYTES OUT_JSON
+    python -m tressfx_amd.profile_summary digest
 
 Counter values are KiB per dispatch.  The copy kernel of the same run
 (__amd_rocclr_copyBuffer, COPY_BYTES read and written) gives the scale factor
 counter_bytes / true_bytes for each counter; a kernel's traffic is its counter
-bytes divided by that factor.
+bytes divided by that factor.  A traffic record carries the digest of the
+fused kernel's sources it was measured on, so a reader can tell whether it
+still describes the kernel that is built.
 """
 import csv
+import hashlib
 import json
+import os
 import statistics
 import sys
+
+FUSED_SOURCES = ("csrc/tfx_fused_kernel.hip", "csrc/tfx_device.h")
+
+
+def fused_source_digest():
+    """sha256 over the sources that define k_fused_substep."""
+    h = hashlib.sha256()
+    here = os.path.dirname(os.path.abspath(__file__))
+    for rel in FUSED_SOURCES:
+        with open(os.path.join(here, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
 
 
 def _rows(path):
@@ -65,6 +82,7 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
     res["bytes_per_launch"] = res["FETCH_SIZE"]["bytes_per_dispatch"] + res["WRITE_SIZE"]["bytes_per_dispatch"]
     res["kernel"] = kernel
     res["copy_bytes"] = copy_bytes
+    res["source_digest"] = fused_source_digest()
     return res
 
 
@@ -77,6 +95,8 @@ def main(argv):
         with open(argv[5], "w") as f:
             json.dump(r, f, indent=1)
         print(json.dumps(r))
+    elif argv[:1] == ["digest"]:
+        print(fused_source_digest())
     else:
         print(__doc__)
         return 2

@@ -146,15 +146,27 @@ def head_yaw(t, amplitude_deg=30.0, freq_hz=1.0):
 # ---------------------------------------------------------------------------
 # Collision meshes
 # ---------------------------------------------------------------------------
-def icosphere(subdivisions):
-    """Unit icosphere, outward counter-clockwise winding. 20*4^k triangles."""
+def _icosahedron():
     t = (1.0 + 5 ** 0.5) / 2.0
     verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
              (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
     faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
              (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
              (6, 2, 10), (8, 6, 7), (9, 8, 1)]
-    v = [np.array(p, float) / np.linalg.norm(p) for p in verts]
+    return [np.array(p, float) / np.linalg.norm(p) for p in verts], faces
+
+
+def _outward(V, Fc):
+    """Flip faces so that each winds counter-clockwise seen from outside."""
+    n = np.cross(V[Fc[:, 1]] - V[Fc[:, 0]], V[Fc[:, 2]] - V[Fc[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[Fc[:, 0]]) < 0
+    Fc[flip] = Fc[flip][:, [0, 2, 1]]
+    return Fc
+
+
+def icosphere(subdivisions):
+    """Unit icosphere, outward counter-clockwise winding. 20*4^k triangles."""
+    v, faces = _icosahedron()
     for _ in range(subdivisions):
         cache = {}
 
@@ -172,12 +184,49 @@ def icosphere(subdivisions):
             nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         faces = nf
     V = np.array(v)
-    Fc = np.array(faces, dtype=np.uint32)
-    # make every face wind counter-clockwise seen from outside
-    n = np.cross(V[Fc[:, 1]] - V[Fc[:, 0]], V[Fc[:, 2]] - V[Fc[:, 0]])
-    flip = np.einsum("ij,ij->i", n, V[Fc[:, 0]]) < 0
-    Fc[flip] = Fc[flip][:, [0, 2, 1]]
-    return V, Fc
+    return V, _outward(V, np.array(faces, dtype=np.uint32))
+
+
+def geodesic_sphere(freq):
+    """Unit geodesic sphere of frequency `freq`: every icosahedron face split
+    into freq^2 triangles on a barycentric lattice and projected onto the
+    sphere (20 freq^2 triangles, 10 freq^2 + 2 vertices), outward
+    counter-clockwise winding.  A lattice point is identified by its integer
+    barycentric weights on the icosahedron's vertex ids, so points shared by
+    neighbouring faces are one vertex with bit-identical coordinates."""
+    base, faces = _icosahedron()
+    base = np.array(base)
+    k = int(freq)
+    ii, jj = np.meshgrid(np.arange(k + 1), np.arange(k + 1), indexing="ij")
+    keep = ii + jj <= k
+    ii, jj = ii[keep], jj[keep]
+    lid = -np.ones((k + 1, k + 1), np.int64)          # lattice (i, j) -> point within a face
+    lid[ii, jj] = np.arange(ii.size)
+    keys = []
+    for a, b, c in faces:
+        ids = np.array([a, b, c])
+        w = np.stack([k - ii - jj, ii, jj], 1)
+        vid = np.where(w > 0, ids[None, :], -1)        # a zero weight does not name its vertex
+        w = np.where(w > 0, w, 0)
+        order = np.argsort(vid, axis=1, kind="stable")
+        keys.append(np.concatenate([np.take_along_axis(vid, order, 1), np.take_along_axis(w, order, 1)], 1))
+    keys = np.concatenate(keys)
+    uniq, inv = np.unique(keys, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    pts = np.zeros((uniq.shape[0], 3))
+    for q in range(3):
+        vq, wq = uniq[:, q], uniq[:, 3 + q]
+        pts += np.where((vq >= 0)[:, None], base[np.maximum(vq, 0)] * wq[:, None], 0.0)
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    # triangles of one face's lattice: up (i,j),(i+1,j),(i,j+1); down (i+1,j),(i+1,j+1),(i,j+1)
+    ui, uj = np.nonzero(np.add.outer(np.arange(k), np.arange(k)) < k)
+    up = np.stack([lid[ui, uj], lid[ui + 1, uj], lid[ui, uj + 1]], 1)
+    di, dj = np.nonzero(np.add.outer(np.arange(k), np.arange(k)) < k - 1)
+    down = np.stack([lid[di + 1, dj], lid[di + 1, dj + 1], lid[di, dj + 1]], 1)
+    local = np.concatenate([up, down])
+    npf = ii.size
+    tri = np.concatenate([inv[f * npf + local] for f in range(len(faces))]).astype(np.uint32)
+    return pts, _outward(pts, tri)
 
 
 class ValueNoise3:
@@ -350,16 +399,22 @@ def yaw_pitch_matrices(yaw, pitch):
 
 
 def trajectory_bones(bone_fn, times):
-    """bone_fn at every time of `times` (float32[..., 16]); vectorised for
-    rigid_trajectory and head_yaw."""
+    """bone_fn at every time of `times`: float32[..., 16] for one bone,
+    float32[..., nb, 16] for several; vectorised for rigid_trajectory and
+    FurBody."""
     times = np.asarray(times, np.float64)
     params = getattr(bone_fn, "params", None)
     if params is not None:
         amp, hz, ph = params
         return yaw_pitch_matrices(amp[0] * np.sin(2.0 * np.pi * hz[0] * times + ph[0]),
                                   amp[1] * np.sin(2.0 * np.pi * hz[1] * times + ph[1]))
-    flat = np.stack([np.asarray(bone_fn(float(t)), np.float32).reshape(16) for t in times.reshape(-1)])
-    return flat.reshape(times.shape + (16,))
+    vec = getattr(getattr(bone_fn, "__self__", None), "bone_matrices", None)
+    if vec is not None:
+        return vec(times)
+    flat = np.stack([np.asarray(bone_fn(float(t)), np.float32).reshape(-1, 16) for t in times.reshape(-1)])
+    if flat.shape[1] == 1:
+        return flat.reshape(times.shape + (16,))
+    return flat.reshape(times.shape + flat.shape[1:])
 
 
 CONFIG3_LENGTHS = (16, 32, 64)
@@ -422,3 +477,119 @@ def config3(num_instances=256, seed=0, workers=1, **kw):
     else:
         assets = [config3_asset(*j) for j in jobs]
     return [config3_instance(i, G, V, n, seed, asset=a) for (i, (G, V, n)), a in zip(enumerate(spec), assets)]
+
+
+# ---------------------------------------------------------------------------
+# Config 5: fur on a skinned body
+# ---------------------------------------------------------------------------
+FUR_SEMI_AXES = (0.25, 0.15, 0.15)                  # a 0.5 x 0.3 x 0.3 m body
+FUR_BONE_X = (-0.1875, -0.0625, 0.0625, 0.1875)     # bone pivots along the body's long axis
+FUR_BONE_SPACING = 0.125
+
+
+def fur_skin_weights(x):
+    """Skinning of a point at body coordinate x to the 4 bones: hat functions
+    centred on the bone pivots (a partition of unity, x clamped to the end
+    pivots).  float32[n, 8]: bone ids as float, then weights."""
+    xc = np.clip(np.asarray(x, np.float64), FUR_BONE_X[0], FUR_BONE_X[-1])
+    w = np.maximum(0.0, 1.0 - np.abs(xc[:, None] - np.asarray(FUR_BONE_X)[None, :]) / FUR_BONE_SPACING)
+    out = np.zeros((xc.shape[0], 8), np.float32)
+    out[:, :4] = np.arange(4, dtype=np.float32)
+    out[:, 4:] = w
+    return out
+
+
+class FurBody:
+    """The config-5 collision body: a closed geodesic sphere of 20 freq^2
+    triangles (freq 100: 200,000) scaled to semi-axes 0.25 x 0.15 x 0.15 m and
+    displaced radially by a static 3-octave value noise of amplitude `amp`,
+    skinned to 4 bones spaced along x.  Bone b bends the body about its pivot
+    (FUR_BONE_X[b], 0, 0) with independent yaw and pitch sinusoids (amplitude
+    U(5, 20) degrees, frequency U(0.3, 2) Hz, random phase)."""
+
+    def __init__(self, rng, freq=100, amp=0.01, noise_freq=6.0):
+        unit, self.indices = geodesic_sphere(freq)
+        noise = ValueNoise3(rng)
+        disp = np.zeros(unit.shape[0])
+        norm = 0.0
+        for o in range(3):
+            a = 0.5 ** o
+            disp += a * noise(unit * noise_freq * (2 ** o) + 17.0 * o)
+            norm += a
+        p = unit * np.asarray(FUR_SEMI_AXES)[None, :] + (amp * disp / norm)[:, None] * unit
+        tri = self.indices.astype(np.int64)
+        fn = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
+        vn = np.zeros_like(p)
+        for q in range(3):
+            np.add.at(vn, tri[:, q], fn)
+        vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+        self.positions = p.astype(np.float32)
+        self.normals = vn.astype(np.float32)
+        self.face_normals = fn / np.linalg.norm(fn, axis=1, keepdims=True)
+        self.face_areas = 0.5 * np.linalg.norm(fn, axis=1)
+        self.skinning = fur_skin_weights(p[:, 0])
+        self.amp = np.radians(rng.uniform(5.0, 20.0, (4, 2)))
+        self.hz = rng.uniform(0.3, 2.0, (4, 2))
+        self.ph = rng.uniform(0.0, 2.0 * math.pi, (4, 2))
+
+    @property
+    def num_triangles(self):
+        return self.indices.shape[0]
+
+    def bone_matrices(self, times):
+        """float32[..., 4, 16] at every time of `times`: bone b is the
+        rotation R_y(yaw) R_x(pitch) about its pivot (row-vector convention,
+        p' = (p - c) R + c)."""
+        t = np.asarray(times, np.float64)[..., None]
+        yaw = self.amp[:, 0] * np.sin(2.0 * np.pi * self.hz[:, 0] * t + self.ph[:, 0])
+        pitch = self.amp[:, 1] * np.sin(2.0 * np.pi * self.hz[:, 1] * t + self.ph[:, 1])
+        m = yaw_pitch_matrices(yaw, pitch).astype(np.float64).reshape(t.shape[:-1] + (4, 4, 4))
+        c = np.zeros((4, 3))
+        c[:, 0] = FUR_BONE_X
+        r = m[..., :3, :3]
+        m[..., 3, :3] = c - np.einsum("bi,...bij->...bj", c, r)
+        return m.reshape(t.shape[:-1] + (4, 16)).astype(np.float32)
+
+    def bone_fn(self, t):
+        return self.bone_matrices(t)
+
+    def skinned_mesh(self):
+        """The rest mesh with its skinning, as TressFXBoneSkinning takes it."""
+        return dict(positions=self.positions, normals=self.normals, skinning=self.skinning, indices=self.indices,
+                    follow_bone=0)
+
+    def sample_surface(self, n, rng):
+        """n area-uniform points on the rest surface with their face normals."""
+        tri = rng.choice(self.indices.shape[0], size=n, p=self.face_areas / self.face_areas.sum())
+        r1, r2 = rng.uniform(0.0, 1.0, n), rng.uniform(0.0, 1.0, n)
+        s1 = np.sqrt(r1)
+        w = np.stack([1.0 - s1, s1 * (1.0 - r2), s1 * r2], 1)
+        p = self.positions.astype(np.float64)[self.indices[tri].astype(np.int64)]
+        return np.einsum("nk,nkj->nj", w, p), self.face_normals[tri]
+
+
+def settings_config5():
+    """Config-1 solver settings with length iterations 2 and VSP 0.7."""
+    s = settings_config1()
+    s.length_constraints_iterations = 2
+    s.vsp_coeff = 0.7
+    return s
+
+
+def config5(num_guides=1_000_000, V=8, F=2, freq=100, grid_n=256, cell=0.0025, follow_sigma=0.0005,
+            seed_instance=0, body=None):
+    """Fur stress: num_guides x V strands (0.002 m segments, the same curl
+    generation) rooted area-uniformly on the 4-bone skinned FurBody, F follow
+    strands per guide (offsets N(0, follow_sigma)), each guide skinned with the
+    body's weights at its root; 256^3 SDF of 2.5 mm cells centred on the body
+    rebuilt every frame from the skinned mesh; 3 substeps."""
+    rng = rng_for(5, seed_instance)
+    if body is None:
+        body = FurBody(rng, freq)
+    roots, nrm = body.sample_surface(num_guides, rng)
+    strands = grow_strands(roots, nrm, V, 0.002, 0.15, rng)
+    skin = np.repeat(fur_skin_weights(roots[:, 0]), F + 1, axis=0)
+    asset = build_asset(strands, F, rng, follow_sigma=follow_sigma, skin=skin)
+    half = 0.5 * grid_n * cell
+    grid = ((-half, -half, -half), cell, (grid_n, grid_n, grid_n))
+    return Workload(asset, settings_config5(), 3, body.bone_fn, mesh=body, sdf_grid=grid)
