@@ -19,6 +19,51 @@ namespace tfx {
 
 struct f3 { float x, y, z; };
 
+// Address-space qualifiers apply to the device pass; the host pass, which
+// only type-checks device functions, sees plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TFX_ADDRSPACE(n) __attribute__((address_space(n)))
+#else
+#define TFX_ADDRSPACE(n)
+#endif
+
+// Loads and stores through the global address space: pointers that come out
+// of the object tables are generic, and generic (flat) accesses also count
+// against the LDS wait counter, so every wait for one of them waits for LDS
+// traffic too.
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+    return *(const TFX_ADDRSPACE(1) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, const T& v) {
+    *(TFX_ADDRSPACE(1) T*)p = v;
+}
+// Host-written records a kernel only reads (object tables, per-substep
+// parameters): read through the constant address space from a wave-uniform
+// address, they become scalar loads that the compiler may hoist and reuse.
+template <typename T>
+using const_as = const TFX_ADDRSPACE(4) T;
+template <typename T>
+__device__ __forceinline__ const_as<T>* uniform_const(const T* p) {
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return (const_as<T>*)(((uint64_t)hi << 32) | lo);
+}
+// A whole record copied out of such memory, word by word.
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized record");
+    const_as<uint32_t>* src = (const_as<uint32_t>*)uniform_const(p);
+    T out;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+    return out;
+}
+__device__ __forceinline__ float4 ld4(const_as<float4>& v) { return make_float4(v.x, v.y, v.z, v.w); }
+
 __device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 xyz(const float4& a) { return f3{a.x, a.y, a.z}; }
 __device__ __forceinline__ f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -133,25 +178,32 @@ __device__ __forceinline__ const float* bone_ptr(const float* bones, float fidx)
 }
 
 // Blends the 4x4 matrices element-wise (the unused fourth column is skipped:
-// no result of the path reads it).
+// no result of the path reads it).  All four matrices are read up front.  A
+// slot of weight <= 0 is not blended, and it reads slot 0's matrix instead of
+// its own: a packed bone table may hold only the bones that slot 0 and the
+// weighted slots name.
 __device__ __forceinline__ void blend_bones(const tfx_bone_skinning_data& sd, const float* bones, BlendedBone& b) {
+    float mat[4][12];
     const float* m0 = bone_ptr(bones, sd.bone_index[0]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float* mi = (i == 0 || !(sd.bone_weight[i] > 0.0f)) ? m0 : bone_ptr(bones, sd.bone_index[i]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mat[i][r * 3 + c] = ldg(&mi[4 * r + c]);
+    }
     const float w0 = sd.bone_weight[0];
     float acc[12];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[r * 3 + c] = m0[4 * r + c] * w0;
+    for (int k = 0; k < 12; ++k) acc[k] = mat[0][k] * w0;
     float wsum = w0;
 #pragma unroll
     for (int i = 1; i < 4; ++i) {
         const float wi = sd.bone_weight[i];
         if (wi > 0.0f) {
-            const float* mi = bone_ptr(bones, sd.bone_index[i]);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) acc[r * 3 + c] = acc[r * 3 + c] + mi[4 * r + c] * wi;
+            for (int k = 0; k < 12; ++k) acc[k] = acc[k] + mat[i][k] * wi;
             wsum = wsum + wi;
         }
     }
@@ -354,7 +406,7 @@ __device__ __forceinline__ bool sdf_brick_of(const SdfGrid& g, f3 p, int& b) {
 __device__ __forceinline__ bool sdf_maybe_near(const SdfGrid& g, f3 p) {
     if (!g.bricks) return true;
     int b;
-    return sdf_brick_of(g, p, b) && g.bricks[b] != 0;
+    return sdf_brick_of(g, p, b) && ldg(&g.bricks[b]) != 0;
 }
 
 template <bool kAllCorners>
@@ -368,11 +420,11 @@ __device__ __forceinline__ float sdf_distance_t(const SdfGrid& g, const float* _
     const int ids[8] = {base, base + X, base + Y, base + Y + X, base + Z, base + Z + X, base + Z + Y, base + Z + Y + X};
     float d[8];
     if (!kAllCorners) {
-        d[0] = sdf[ids[0]];
+        d[0] = ldg(&sdf[ids[0]]);
         if (d[0] == INITIAL) return INITIAL;
     }
 #pragma unroll
-    for (int j = kAllCorners ? 0 : 1; j < 8; ++j) d[j] = sdf[ids[j]];
+    for (int j = kAllCorners ? 0 : 1; j < 8; ++j) d[j] = ldg(&sdf[ids[j]]);
     bool outside = false;
 #pragma unroll
     for (int j = 0; j < 8; ++j) outside |= (d[j] == INITIAL);
@@ -432,7 +484,9 @@ __device__ __forceinline__ void sdf_collide_batch(const SdfGrid& g, const float*
 
 __device__ __forceinline__ void store_xyz(float4* p, f3 v) {
     float* f = reinterpret_cast<float*>(p);
-    f[0] = v.x; f[1] = v.y; f[2] = v.z;
+    stg(&f[0], v.x);
+    stg(&f[1], v.y);
+    stg(&f[2], v.z);
 }
 
 }  // namespace tfx

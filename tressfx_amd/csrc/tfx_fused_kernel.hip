@@ -64,8 +64,8 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
         const WorkItem item = items[w];
         const int hi = __builtin_amdgcn_readfirstlane(item.hair);
         const int g0 = __builtin_amdgcn_readfirstlane(item.g0);
-        const DevHair h = hairs[hi];
-        const DevParams& P = *h.params;
+        const DevHair h = load_uniform(hairs + hi);
+        const_as<DevParams>& P = *uniform_const(h.params);
         const int F = h.F;
         const int nb = min(B, h.G - g0);
 
@@ -74,28 +74,44 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
         const int brick_words = (P.sdf_bx * P.sdf_by * P.sdf_bz + 31) >> 5;
         const bool bits_lds = has_sdf && h.sdf_brick_bits != nullptr && brick_words <= kMaxBrickWords;
         if (bits_lds)
-            for (int k = t; k < brick_words; k += kFusedBlock) sBits[k] = h.sdf_brick_bits[k];
+            for (int k = t; k < brick_words; k += kFusedBlock) sBits[k] = ldg(&h.sdf_brick_bits[k]);
 
         // ---- P0: blended bone per strand ---------------------------------
         for (int s = tt; s < nb; s += kFusedBlock) {
             const size_t strand = (size_t)(g0 + s) * (F + 1);
             BlendedBone bb;
-            blend_bones(h.skin[strand], h.bones, bb);
+            blend_bones(ldg(&h.skin[strand]), h.bones, bb);
 #pragma unroll
             for (int k = 0; k < 9; ++k) sMat[s][k] = bb.m[k];
 #pragma unroll
             for (int k = 0; k < 3; ++k) sMat[s][9 + k] = bb.t[k];
             sSkinQ[s] = quat_from_matrix3(bb.m);
             if (F <= kMaxStagedFollow)
-                for (int j = 0; j < F; ++j) sOff[s][j] = h.follow_off[strand + 1 + j];
+                for (int j = 0; j < F; ++j) sOff[s][j] = ldg(&h.follow_off[strand + 1 + j]);
         }
         __syncthreads();
 
         // ---- P1: integration and global shape ------------------------------
+        // Every global load of the phase is issued before any of its results
+        // is used (lanes past the block's last strand read strand g0's
+        // vertices and discard them).
         float4 prevReg[NPT];
+        float restLen[NPT];
+        float4 restIn[NPT], posIn[NPT], prevIn[NPT];
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int lv = k * kFusedBlock + t;
+            const int ls = lv / V, v = lv % V;
+            const size_t i = ((size_t)(g0 + (ls < nb ? ls : 0)) * (F + 1)) * V + v;
+            restIn[k] = ldg(&h.rest[i]);
+            posIn[k] = ldg(&h.pos[i]);
+            prevIn[k] = ldg(&h.prev[i]);
+            restLen[k] = ldg(&h.restlen[i]);
+        }
         const bool reset = P.reset != 0.0f;
         const float grav = P.gravity, dt = P.dt, decay = P.decay;
         const bool gshape = P.global_stiffness > 0.0f && P.global_range != 0.0f;
+        const float gstiff = P.global_stiffness, grange = P.global_range;
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
             const int lv = k * kFusedBlock + t;
@@ -103,7 +119,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
             prevReg[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             const bool valid = ls < nb;
             const size_t i = ((size_t)(g0 + (valid ? ls : 0)) * (F + 1)) * V + v;
-            float4 initialPos = valid ? h.rest[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 initialPos = valid ? restIn[k] : make_float4(0.f, 0.f, 0.f, 0.f);
             // Bind pose of LocalShapeConstraints, MultQuaternionAndVector(boneQuat,
             // rest) (:799-801); a strand's vertices sit in consecutive lanes of one
             // wave (64 % V == 0), so the neighbours come from lane +-1.
@@ -116,8 +132,8 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 if (v >= 1 && v < V - 1) nbp = normalize3(sub(bind, bindM));
                 sVB[ls * STRIDE + v] = make_float4(vb.x, vb.y, vb.z, nbp.x);
                 sNB[ls * STRIDE + v] = make_float2(nbp.y, nbp.z);
-                float4 shared = h.pos[i];
-                const float4 prevOld = h.prev[i];
+                float4 shared = posIn[k];
+                const float4 prevOld = prevIn[k];
                 float m[9], tr[3];
 #pragma unroll
                 for (int q = 0; q < 9; ++q) m[q] = sMat[ls][q];
@@ -136,14 +152,14 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 } else {
                     shared = initialPos;
                 }
-                if (gshape && movable(shared) && (float)v < P.global_range * (float)V) {
-                    f3 del = mul(P.global_stiffness, sub(xyz(initialPos), xyz(shared)));
+                if (gshape && movable(shared) && (float)v < grange * (float)V) {
+                    f3 del = mul(gstiff, sub(xyz(initialPos), xyz(shared)));
                     shared.x += del.x; shared.y += del.y; shared.z += del.z;
                 }
                 sPos[ls * STRIDE + v] = shared;
                 prevReg[k] = currentPos;
                 if (v < 2) {
-                    h.prevprev[i] = prevOld;
+                    stg(&h.prevprev[i], prevOld);
                     sV01[ls][v] = currentPos;
                     if (v == 1) sV01[ls][2] = prevOld;
                 }
@@ -256,8 +272,8 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 if (ls < nb && v >= 2 && v < V - 1) {
                     const int strand = (g0 + ls) * (F + 1);
                     float a = (float)(uint32_t)(strand % 20) / 20.0f;
-                    f3 wv = add(add(add(mul(a, xyz(P.wind0)), mul(1.0f - a, xyz(P.wind1))), mul(a, xyz(P.wind2))),
-                                mul(1.0f - a, xyz(P.wind3)));
+                    f3 wv = add(add(add(mul(a, xyz(ld4(P.wind0))), mul(1.0f - a, xyz(ld4(P.wind1)))), mul(a, xyz(ld4(P.wind2)))),
+                                mul(1.0f - a, xyz(ld4(P.wind3))));
                     f3 d = sub(xyz(sPos[ls * STRIDE + v]), xyz(sPos[ls * STRIDE + v + 1]));
                     f3 c = cross3(cross3(d, wv), d);
                     force[k] = f3{-c.x, -c.y, -c.z};
@@ -275,14 +291,6 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 }
             }
             __syncthreads();
-        }
-        float restLen[NPT];
-#pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            const int lv = k * kFusedBlock + t;
-            const int ls = lv / V, v = lv % V;
-            restLen[k] = 0.0f;
-            if (ls < nb) restLen[k] = h.restlen[((size_t)(g0 + ls) * (F + 1)) * V + v];
         }
         for (int it = 0; it < P.length_iters; ++it) {
 #pragma unroll
@@ -318,7 +326,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                     float4 cur = sPos[ls * STRIDE + v];
                     for (int c = 0; c < P.num_capsules; ++c) {
                         f3 np;
-                        if (capsule_collision(cur, prevReg[k], np, P.cap0[c], P.cap1[c])) {
+                        if (capsule_collision(cur, prevReg[k], np, ld4(P.cap0[c]), ld4(P.cap1[c]))) {
                             cur.x = np.x; cur.y = np.y; cur.z = np.z;
                             hitMask |= 1u << k;
                         }
@@ -397,14 +405,14 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                         pout.x = outp[u].x; pout.y = outp[u].y; pout.z = outp[u].z;
                         pv.x = outp[u].x; pv.y = outp[u].y; pv.z = outp[u].z;
                     }
-                    h.pos[i] = pout;
-                    h.prev[i] = pv;
+                    stg(&h.pos[i], pout);
+                    stg(&h.prev[i], pv);
                     float tw;
                     if (tw_uniform) {
                         tw = P.tangent_w;
-                        h.tangent[i] = make_float4(tan.x, tan.y, tan.z, tw);
+                        stg(&h.tangent[i], make_float4(tan.x, tan.y, tan.z, tw));
                     } else {
-                        tw = h.tangent[i].w;
+                        tw = ldg(&h.tangent[i].w);
                         store_xyz(&h.tangent[i], tan);
                     }
                     // sVB is dead after the LSC sweep: it now carries the
@@ -438,7 +446,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 const float4 sp = sPos[ls * STRIDE + v];
                 ft = sVB[ls * STRIDE + v];
                 const float factor = tip * ((float)v / (float)V) + 1.0f;
-                const f3 off = xyz(staged ? sOff[ls][j - 1] : h.follow_off[strand + j]);
+                const f3 off = xyz(staged ? sOff[ls][j - 1] : ldg(&h.follow_off[strand + j]));
                 fp = add(xyz(sp), mul(factor, off));
             };
             if (staged) {
@@ -452,7 +460,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                         queue[atomicAdd(&sQueueCount, 1)] = (uint16_t)idx;
                     } else {
                         store_xyz(&h.pos[fi], fp);
-                        h.tangent[fi] = ft;
+                        stg(&h.tangent[fi], ft);
                     }
                 }
                 __syncthreads();
@@ -483,7 +491,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                     if (!ok[u]) continue;
                     if (hit[u]) store_xyz(&h.prev[fi[u]], outp[u]);
                     store_xyz(&h.pos[fi[u]], hit[u] ? outp[u] : fp[u]);
-                    h.tangent[fi[u]] = ft[u];
+                    stg(&h.tangent[fi[u]], ft[u]);
                 }
             }
         }
