@@ -133,11 +133,67 @@ __device__ __forceinline__ void finalize_cell(uint32_t* __restrict__ cells, int 
             for (int bx = x0; bx <= x1; ++bx) bricks[(bz * g.by + by) * g.bx + bx] = 1;
 }
 
+// Finalize of the four nodes 4q..4q+3, one quad of an x-row (nx % 4 == 0):
+// the same result as finalize_cell on each.  Each touched node x marks the
+// bricks (x-2)>>2 .. (x+2)>>2 along x, and every such range holds the quad's
+// own brick, so their union is the range of the first and last touched node.
+__device__ __forceinline__ void finalize_quad(uint32_t* __restrict__ cells, int q, const SdfGrid& g, float inv_nq,
+                                              float inv_ny, bool fast_index) {
+    uint4* p = reinterpret_cast<uint4*>(cells) + q;
+    const uint4 u = ldg(p);
+    stg(p, make_uint4(ifloat_flip3(u.x), ifloat_flip3(u.y), ifloat_flip3(u.z), ifloat_flip3(u.w)));
+    const uint32_t far = float_flip3(1e10f);
+    const int m = (u.x != far ? 1 : 0) | (u.y != far ? 2 : 0) | (u.z != far ? 4 : 0) | (u.w != far ? 8 : 0);
+    if (m == 0 || !g.bricks) return;
+    const int nq = g.nx >> 2;
+    int xq, y, z, r;
+    if (fast_index) {
+        divmod24(q, nq, inv_nq, r, xq);
+        divmod24(r, g.ny, inv_ny, z, y);
+    } else {
+        xq = q % nq;
+        r = q / nq;
+        y = r % g.ny;
+        z = r / g.ny;
+    }
+    const int xlo = 4 * xq + __builtin_ctz(m), xhi = 4 * xq + 31 - __builtin_clz(m);
+    const int x0 = max(xlo - kBrickHalo, 0) >> kBrickShift, x1 = min((xhi + kBrickHalo) >> kBrickShift, g.bx - 1);
+    const int y0 = max(y - kBrickHalo, 0) >> kBrickShift, y1 = min((y + kBrickHalo) >> kBrickShift, g.by - 1);
+    const int z0 = max(z - kBrickHalo, 0) >> kBrickShift, z1 = min((z + kBrickHalo) >> kBrickShift, g.bz - 1);
+    uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
+    for (int bz = z0; bz <= z1; ++bz)
+        for (int by = y0; by <= y1; ++by)
+            for (int bx = x0; bx <= x1; ++bx) bricks[(bz * g.by + by) * g.bx + bx] = 1;
+}
+
+// Whole-grid passes over `ncell` nodes: four nodes per lane and iteration
+// when the rows split into quads, else one.
+__device__ __forceinline__ void init_cells(uint32_t* __restrict__ sdf, int ncell, int lane, int stride) {
+    const uint32_t v = float_flip3(1e10f);
+    if ((ncell & 3) == 0) {
+        uint4* p = reinterpret_cast<uint4*>(sdf);
+        for (int q = lane; q < (ncell >> 2); q += stride) stg(&p[q], make_uint4(v, v, v, v));
+    } else {
+        for (int i = lane; i < ncell; i += stride) stg(&sdf[i], v);
+    }
+}
+
+__device__ __forceinline__ void finalize_cells(uint32_t* __restrict__ sdf, int ncell, const SdfGrid& g, int lane,
+                                               int stride) {
+    const bool fast = ncell <= (1 << 24);
+    if ((g.nx & 3) == 0) {
+        const float inv_nq = 1.0f / (float)(g.nx >> 2), inv_ny = 1.0f / (float)g.ny;
+        for (int q = lane; q < (ncell >> 2); q += stride) finalize_quad(sdf, q, g, inv_nq, inv_ny, fast);
+    } else {
+        const float inv_nx = 1.0f / (float)g.nx, inv_ny = 1.0f / (float)g.ny;
+        for (int i = lane; i < ncell; i += stride) finalize_cell(sdf, i, g, inv_nx, inv_ny, fast);
+    }
+}
+
 // InitializeSignedDistanceField :301-312, plus clearing the brick occupancy.
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_init(uint32_t* __restrict__ sdf, int ncell,
                                                         uint8_t* __restrict__ bricks, int nbrick) {
-    const uint32_t v = float_flip3(1e10f);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock) sdf[i] = v;
+    init_cells(sdf, ncell, blockIdx.x * kSdfBlock + threadIdx.x, gridDim.x * kSdfBlock);
     if (bricks)
         for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < nbrick; i += gridDim.x * kSdfBlock) bricks[i] = 0;
 }
@@ -145,10 +201,7 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_init(uint32_t* __restrict__ s
 // FinalizeSignedDistanceField :400-412.  A cell that ends up closer than 1e10
 // also marks every brick whose kBrickHalo-dilated extent contains it.
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize(uint32_t* __restrict__ sdf, int ncell, SdfGrid g) {
-    const float inv_nx = 1.0f / (float)g.nx, inv_ny = 1.0f / (float)g.ny;
-    const bool fast = ncell <= (1 << 24);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < ncell; i += gridDim.x * kSdfBlock)
-        finalize_cell(sdf, i, g, inv_nx, inv_ny, fast);
+    finalize_cells(sdf, ncell, g, blockIdx.x * kSdfBlock + threadIdx.x, gridDim.x * kSdfBlock);
 }
 
 // Brick occupancy packed to one bit per brick (word w covers bricks 32w..32w+31).
@@ -250,7 +303,8 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_collide(float4* __restrict__ 
 extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
                                            const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream) {
     using namespace tfx;
-    const int cblocks = std::min((ncell + kSdfBlock - 1) / kSdfBlock, 4096);
+    // init and finalize take four nodes per lane where the grid allows it
+    const int cblocks = std::max(1, std::min((ncell / 4 + kSdfBlock - 1) / kSdfBlock, 4096));
     uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
     const int nbrick = g.bx * g.by * g.bz;
     if (phase == 0) {
@@ -296,8 +350,7 @@ namespace tfx {
 
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_init_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
-    const uint32_t v = float_flip3(1e10f);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < j.ncell; i += gridDim.x * kSdfBlock) j.cells[i] = v;
+    init_cells(j.cells, j.ncell, blockIdx.x * kSdfBlock + threadIdx.x, gridDim.x * kSdfBlock);
     const int nbrick = j.g.bx * j.g.by * j.g.bz;
     uint8_t* bricks = const_cast<uint8_t*>(j.g.bricks);
     for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < nbrick; i += gridDim.x * kSdfBlock) bricks[i] = 0;
@@ -314,11 +367,7 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob*
 
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
-    const SdfGrid g = j.g;
-    const float inv_nx = 1.0f / (float)g.nx, inv_ny = 1.0f / (float)g.ny;
-    const bool fast = j.ncell <= (1 << 24);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < j.ncell; i += gridDim.x * kSdfBlock)
-        finalize_cell(j.cells, i, g, inv_nx, inv_ny, fast);
+    finalize_cells(j.cells, j.ncell, j.g, blockIdx.x * kSdfBlock + threadIdx.x, gridDim.x * kSdfBlock);
 }
 
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_multi(const SdfJob* __restrict__ jobs) {
@@ -340,7 +389,7 @@ extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, i
                                            hipStream_t stream) {
     using namespace tfx;
     if (njobs <= 0) return hipSuccess;
-    const int cx = std::max(1, std::min((max_cells + kSdfBlock - 1) / kSdfBlock, 1024));
+    const int cx = std::max(1, std::min((max_cells / 4 + kSdfBlock - 1) / kSdfBlock, 1024));
     switch (phase) {
         case 0: hipLaunchKernelGGL(k_sdf_init_multi, dim3(cx, njobs), dim3(kSdfBlock), 0, stream, jobs); break;
         case 1: {
