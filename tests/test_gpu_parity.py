@@ -273,6 +273,39 @@ def test_fused_multi_object_mixed_lengths(gpu, variant):
                     assert mm is None, f"V={wl.asset.num_vertices_per_strand} {k}: {mm}"
 
 
+@pytest.mark.parametrize("iters", [1, 3, 5, 7])
+@pytest.mark.parametrize("V", [16, 32])
+def test_fused_local_shape_iteration_counts(gpu, V, iters):
+    """The pipelined local-shape sweep (up to four passes per group, one lane
+    each) with pass counts that leave idle lanes or need several groups, on
+    enough strands that they straddle wave boundaries: fused equals plain
+    (the reference re-dispatch) every substep, and plain equals the FP32
+    transcription on the first."""
+    print(f"[fused lsc] V={V} iterations={iters}", flush=True)
+    wl = small_workload(300, V, 1, seed=60 + iters)
+    wl.settings.local_constraints_iterations = iters
+    a, b = make_hair(wl), make_hair(wl)
+    sim_p, sim_f = tfx.TressFXSimulation(), tfx.TressFXSimulation()
+    o32 = oracle.OracleHair(wl.asset, np.float32)
+    for frame in range(3):
+        for sub in range(2):
+            set_substep_inputs(a, wl, frame, sub)
+            set_substep_inputs(b, wl, frame, sub)
+            if frame == 0 and sub == 0:
+                params, bones = a.simulation_params()
+                assert params.cpu_local_shape_iterations == iters
+                o32.set_state(a.download_state())
+                o32.simulate(params, bones)
+            sim_p.Simulate([a])
+            sim_f.SimulateFused([b])
+            sa, sb = a.download_state(), b.download_state()
+            if frame == 0 and sub == 0:
+                compare_state_fp32(sa, o32, "plain vs FP32 transcription")
+            for k in ("pos", "prev", "tangent"):
+                mm = first_mismatch(sb[k], sa[k])
+                assert mm is None, f"frame {frame} substep {sub}: fused {k} differs from plain at {mm}"
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2])
 def test_fused_sdf_launch_variants(gpu, variant):
     """Fused simulation + SDF collision at every block size equals plain

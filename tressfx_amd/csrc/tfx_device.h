@@ -77,6 +77,30 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
 }
 __device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
 __device__ __forceinline__ f3 normalize3(f3 a) { return divs(a, sqrtf(dot3(a, a))); }
+// a / b, bit-identical to the IEEE quotient, for a divisor b shared by many
+// divisions, given y = 1/b correctly rounded (one IEEE division per divisor):
+// q = a*y, then twice r = a - b*q (exact in an FMA) and q += r*y.  When the
+// result lies in [2^-100, FLT_MAX] this is the correctly rounded quotient
+// (Markstein's correction; the remaining cases, zeros and underflow and
+// overflow and NaNs, are recomputed by IEEE division).
+__device__ __forceinline__ float div_fixed(float a, float b, float y) {
+    float q = a * y;
+    float r = __builtin_fmaf(-q, b, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-q, b, a);
+    q = __builtin_fmaf(r, y, q);
+    if (!(__builtin_fabsf(q) >= 0x1p-100f && __builtin_fabsf(q) <= 0x1.fffffep127f)) {
+        // An empty volatile statement cannot be speculated: it keeps this
+        // rarely taken path a branch instead of a division on every lane.
+        float aa = a;
+        asm volatile("" : "+v"(aa));
+        q = aa / b;
+    }
+    return q;
+}
+__device__ __forceinline__ f3 divs_fixed(f3 a, float b, float y) {
+    return f3{div_fixed(a.x, b, y), div_fixed(a.y, b, y), div_fixed(a.z, b, y)};
+}
 __device__ __forceinline__ float fmn(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ float fmx(float a, float b) { return fmaxf(a, b); }
 
@@ -296,7 +320,8 @@ struct alignas(16) DevParams {
     int tangent_w_uniform;  // all guide tangent w equal -> tangent_w
     float tangent_w;
     float sdf_inv_cell;
-    int sdf_bx, sdf_by, sdf_bz, pad2;
+    int sdf_bx, sdf_by, sdf_bz;
+    float sdf_inv_h;        // 1 / (0.1 * cell), correctly rounded
 };
 
 // Device pointers and counts of one hair object.
@@ -334,8 +359,9 @@ struct SdfGrid {
     // other than 1e10.  Null = no map (every lookup is exact).
     const uint8_t* bricks;
     const uint32_t* brick_bits; // the same occupancy, one bit per brick
-    float inv_cell;             // 1 / cell, for the conservative brick test only
+    float inv_cell;             // 1 / cell, correctly rounded (the quotients by cell, the brick test)
     int bx, by, bz;             // bricks per axis
+    float inv_h;                // 1 / (0.1 * cell), correctly rounded (the gradient step)
 };
 
 constexpr int kBrickShift = 2;  // 4^3 cells per brick
@@ -363,9 +389,9 @@ struct MeshSkinJob {
 
 // GetSdfCoordinates :103-113
 __device__ __forceinline__ void sdf_coords(const SdfGrid& g, f3 p, int& cx, int& cy, int& cz) {
-    cx = ftoi((p.x - g.ox) / g.cell);
-    cy = ftoi((p.y - g.oy) / g.cell);
-    cz = ftoi((p.z - g.oz) / g.cell);
+    cx = ftoi(div_fixed(p.x - g.ox, g.cell, g.inv_cell));
+    cy = ftoi(div_fixed(p.y - g.oy, g.cell, g.inv_cell));
+    cz = ftoi(div_fixed(p.z - g.oz, g.cell, g.inv_cell));
 }
 // GetSdfCellIndex :123-129 (wrapping int arithmetic)
 __device__ __forceinline__ int sdf_index(const SdfGrid& g, int x, int y, int z) {
@@ -430,7 +456,7 @@ __device__ __forceinline__ float sdf_distance_t(const SdfGrid& g, const float* _
     for (int j = 0; j < 8; ++j) outside |= (d[j] == INITIAL);
     if (outside) return INITIAL;
     f3 cp = sdf_cell_pos(g, cx, cy, cz);
-    f3 t = divs(sub(p, cp), g.cell);
+    f3 t = divs_fixed(sub(p, cp), g.cell, g.inv_cell);
     float a = lerp_(lerp_(d[0], d[1], t.x), lerp_(d[2], d[3], t.x), t.y);
     float b = lerp_(lerp_(d[4], d[5], t.x), lerp_(d[6], d[7], t.x), t.y);
     return lerp_(a, b, t.z);
@@ -445,7 +471,7 @@ __device__ __forceinline__ bool sdf_project(const SdfGrid& g, const float* __res
     float fx = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 1.0f * h, p.y + 0.0f * h, p.z + 0.0f * h});
     float fy = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 0.0f * h, p.y + 1.0f * h, p.z + 0.0f * h});
     float fz = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 0.0f * h, p.y + 0.0f * h, p.z + 1.0f * h});
-    f3 grad = divs(f3{fx - distance, fy - distance, fz - distance}, h);
+    f3 grad = divs_fixed(f3{fx - distance, fy - distance, fz - distance}, h, g.inv_h);
     f3 n = normalize3(grad);
     if (distance < margin) {
         out = add(p, mul(n, margin - distance));

@@ -207,19 +207,24 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
         __syncthreads();
 
         // ---- P4: local shape constraints (Gauss–Seidel sweeps) -------------
-        // Pass k of the sweep at vertex i reads vertices i-1..i+1 and writes
-        // i, i+1; once pass k has completed step i+2 nothing it does later
-        // touches vertices <= i+1.  So up to kLscPipe passes run together, pass
-        // k on its own lane lagging kLscLag steps behind pass k-1: every step
-        // reads exactly the values the sequential order gives it.  The lanes
-        // of one strand are adjacent in one wave; the wave runs the steps in
-        // lock step and orders its LDS accesses by a wave-scope fence per step.
+        // Step i of a pass reads vertex i+1 from LDS (vertices i-1 and i come
+        // from its own previous step) and stores vertex i, whose value is then
+        // final for the pass (the last step also stores vertex V-1).  So up to
+        // kLscPipe passes run together, pass k on its own lane kLscLag = 2
+        // steps behind pass k-1: pass k's step i runs one step after pass k-1
+        // stored vertex i+1, and pass k-1 never reads a vertex that pass k
+        // has stored.  Every step reads exactly the values the sequential
+        // order gives it.  The lanes of one strand are adjacent in one wave;
+        // the wave runs the steps in lock step and orders its LDS accesses by
+        // a wave-scope fence per step.
         if (P.lsc_passes > 0) {
             const float stiff = 0.5f * fmn(P.local_stiffness, 0.95f);
-            constexpr int kLscPipe = 4, kLscLag = 3;
+            constexpr int kLscPipe = 4, kLscLag = 2;
             const int npass = P.lsc_passes;
-            const int lanes = npass < kLscPipe ? npass : kLscPipe;
-            const int steps = (V - 2) + kLscLag * (lanes - 1);
+            // 1, 2 or 4 lanes per strand: a count that divides 64 keeps the
+            // lanes of a strand in one wave (with 3 or more passes the extra
+            // lanes of a group are idle).
+            const int lanes = npass >= 3 ? kLscPipe : npass;
             for (int item = tt; item < nb * lanes; item += kFusedBlock) {
                 const int s = item / lanes, j = item - s * lanes;
                 float4* sp = &sPos[s * STRIDE];
@@ -231,6 +236,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 for (int group = 0; group < npass; group += lanes) {
                     const bool live = group + j < npass;
                     const int lag = kLscLag * j;
+                    const int steps = (V - 2) + kLscLag * (min(lanes, npass - group) - 1);
                     float4 pm = make_float4(0.f, 0.f, 0.f, 0.f), p = pm;
                     for (int it = 1; it <= steps; ++it) {
                         const int lv = it - lag;
@@ -348,6 +354,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
         grid.brick_bits = nullptr;
         grid.inv_cell = P.sdf_inv_cell;
         grid.bx = P.sdf_bx; grid.by = P.sdf_by; grid.bz = P.sdf_bz;
+        grid.inv_h = P.sdf_inv_h;
         auto maybe_near = [&](f3 p) -> bool {
             if (!bits_lds) return true;
             int b;

@@ -24,15 +24,6 @@ __device__ __forceinline__ uint32_t float_flip3(float f) {
 }
 __device__ __forceinline__ uint32_t ifloat_flip3(uint32_t u) { return (u >> 1) | (u << 31); }
 
-// DistancePointToEdge :147-159 (the normal it also forms is unused)
-__device__ __forceinline__ float distance_point_to_edge(f3 p, f3 x0, f3 x1) {
-    f3 x10 = sub(x1, x0);
-    float t = dot3(sub(x1, p), x10) / dot3(x10, x10);
-    t = fmx(0.0f, fmn(t, 1.0f));
-    f3 a = sub(p, add(mul(t, x0), mul(1.0f - t, x1)));
-    return len3(a);
-}
-
 // Per-triangle constants of SignedDistancePointToTriangle :163-215 that do
 // not depend on the query point.
 struct TriPrep {
@@ -41,6 +32,10 @@ struct TriPrep {
     f3 x12n;   // x1 - x2 made orthogonal to x02n, normalised
     float l0, l1, l2;
     f3 nTri;
+    float rl0, rl2;          // 1 / l0, 1 / l2 (correctly rounded)
+    f3 e10, e21, e20;        // edge vectors x1 - x0, x2 - x1, x2 - x0
+    float q10, q21, q20;     // their squared lengths
+    float rq10, rq21, rq20;  // and the reciprocals of those
 };
 
 __device__ __forceinline__ TriPrep tri_prep(f3 x0, f3 x1, f3 x2) {
@@ -57,13 +52,24 @@ __device__ __forceinline__ TriPrep tri_prep(f3 x0, f3 x1, f3 x2) {
     t.x02n = x02;
     t.x12n = x12;
     t.nTri = cross3(sub(x1, x0), sub(x2, x0));
+    t.rl0 = 1.0f / t.l0;
+    t.rl2 = 1.0f / t.l2;
+    t.e10 = sub(x1, x0);
+    t.e21 = sub(x2, x1);
+    t.e20 = sub(x2, x0);
+    t.q10 = dot3(t.e10, t.e10);
+    t.q21 = dot3(t.e21, t.e21);
+    t.q20 = dot3(t.e20, t.e20);
+    t.rq10 = 1.0f / t.q10;
+    t.rq21 = 1.0f / t.q21;
+    t.rq20 = 1.0f / t.q20;
     return t;
 }
 
-// The squared length DistancePointToEdge takes the square root of.
-__device__ __forceinline__ float sq_distance_point_to_edge(f3 p, f3 x0, f3 x1) {
-    f3 x10 = sub(x1, x0);
-    float t = dot3(sub(x1, p), x10) / dot3(x10, x10);
+// The squared length DistancePointToEdge takes the square root of, with the
+// edge x10 = x1 - x0, its squared length q and 1/q precomputed.
+__device__ __forceinline__ float sq_distance_point_to_edge(f3 p, f3 x0, f3 x1, f3 x10, float q, float rq) {
+    float t = div_fixed(dot3(sub(x1, p), x10), q, rq);
     t = fmx(0.0f, fmn(t, 1.0f));
     f3 a = sub(p, add(mul(t, x0), mul(1.0f - t, x1)));
     return dot3(a, a);
@@ -76,8 +82,8 @@ __device__ __forceinline__ float sq_distance_point_to_edge(f3 p, f3 x0, f3 x1) {
 // which fmin drops) with one square root instead of three.
 __device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p) {
     f3 px2 = sub(p, t.x2);
-    float b = dot3(t.x12n, px2) / t.l2;
-    float a = (dot3(t.x02n, px2) - t.l1 * b) / t.l0;
+    float b = div_fixed(dot3(t.x12n, px2), t.l2, t.rl2);
+    float a = div_fixed(dot3(t.x02n, px2) - t.l1 * b, t.l0, t.rl0);
     float c = (1.0f - a) - b;
     const float tol = 1e-8f;
     float d;
@@ -85,9 +91,9 @@ __device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p) {
         f3 n = sub(p, add(add(mul(a, t.x0), mul(b, t.x1)), mul(c, t.x2)));
         d = len3(n);
     } else {
-        float e = sq_distance_point_to_edge(p, t.x0, t.x1);
-        e = fmn(e, sq_distance_point_to_edge(p, t.x1, t.x2));
-        e = fmn(e, sq_distance_point_to_edge(p, t.x0, t.x2));
+        float e = sq_distance_point_to_edge(p, t.x0, t.x1, t.e10, t.q10, t.rq10);
+        e = fmn(e, sq_distance_point_to_edge(p, t.x1, t.x2, t.e21, t.q21, t.rq21));
+        e = fmn(e, sq_distance_point_to_edge(p, t.x0, t.x2, t.e20, t.q20, t.rq20));
         d = sqrtf(e);
     }
     return (dot3(sub(p, t.x0), t.nTri) < 0.0f) ? -d : d;
