@@ -78,18 +78,24 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
 __device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
 __device__ __forceinline__ f3 normalize3(f3 a) { return divs(a, sqrtf(dot3(a, a))); }
 // a / b, bit-identical to the IEEE quotient, for a divisor b shared by many
-// divisions, given y = 1/b correctly rounded (one IEEE division per divisor):
-// q = a*y, then twice r = a - b*q (exact in an FMA) and q += r*y.  When the
-// result lies in [2^-100, FLT_MAX] this is the correctly rounded quotient
-// (Markstein's correction; the remaining cases, zeros and underflow and
-// overflow and NaNs, are recomputed by IEEE division).
+// divisions, given y = recip_fixed(b): q = a*y, then twice r = a - b*q (exact
+// in an FMA) and q += r*y.  With y = 1/b correctly rounded, |a| >= 2^-96 (the
+// residual stays exactly representable) and the result in [2^-100, FLT_MAX]
+// this is the correctly rounded quotient (Markstein's correction); every
+// other case (zeros, underflow, overflow, NaNs, divisors outside
+// [2^-100, 2^100]) is recomputed by IEEE division.
+__host__ __device__ __forceinline__ float recip_fixed(float b) {
+    const float m = b < 0.0f ? -b : b;
+    return (m >= 0x1p-100f && m <= 0x1p100f) ? 1.0f / b : __builtin_nanf("");
+}
 __device__ __forceinline__ float div_fixed(float a, float b, float y) {
     float q = a * y;
     float r = __builtin_fmaf(-q, b, a);
     q = __builtin_fmaf(r, y, q);
     r = __builtin_fmaf(-q, b, a);
     q = __builtin_fmaf(r, y, q);
-    if (!(__builtin_fabsf(q) >= 0x1p-100f && __builtin_fabsf(q) <= 0x1.fffffep127f)) {
+    if (!(__builtin_fabsf(a) >= 0x1p-96f && __builtin_fabsf(q) >= 0x1p-100f &&
+          __builtin_fabsf(q) <= 0x1.fffffep127f)) {
         // An empty volatile statement cannot be speculated: it keeps this
         // rarely taken path a branch instead of a division on every lane.
         float aa = a;
@@ -321,7 +327,7 @@ struct alignas(16) DevParams {
     float tangent_w;
     float sdf_inv_cell;
     int sdf_bx, sdf_by, sdf_bz;
-    float sdf_inv_h;        // 1 / (0.1 * cell), correctly rounded
+    float sdf_inv_h;        // recip_fixed(0.1 * cell)
 };
 
 // Device pointers and counts of one hair object.
@@ -359,9 +365,9 @@ struct SdfGrid {
     // other than 1e10.  Null = no map (every lookup is exact).
     const uint8_t* bricks;
     const uint32_t* brick_bits; // the same occupancy, one bit per brick
-    float inv_cell;             // 1 / cell, correctly rounded (the quotients by cell, the brick test)
+    float inv_cell;             // recip_fixed(cell) (the quotients by cell, the brick test)
     int bx, by, bz;             // bricks per axis
-    float inv_h;                // 1 / (0.1 * cell), correctly rounded (the gradient step)
+    float inv_h;                // recip_fixed(0.1 * cell) (the gradient step)
 };
 
 constexpr int kBrickShift = 2;  // 4^3 cells per brick

@@ -52,17 +52,17 @@ __device__ __forceinline__ TriPrep tri_prep(f3 x0, f3 x1, f3 x2) {
     t.x02n = x02;
     t.x12n = x12;
     t.nTri = cross3(sub(x1, x0), sub(x2, x0));
-    t.rl0 = 1.0f / t.l0;
-    t.rl2 = 1.0f / t.l2;
+    t.rl0 = recip_fixed(t.l0);
+    t.rl2 = recip_fixed(t.l2);
     t.e10 = sub(x1, x0);
     t.e21 = sub(x2, x1);
     t.e20 = sub(x2, x0);
     t.q10 = dot3(t.e10, t.e10);
     t.q21 = dot3(t.e21, t.e21);
     t.q20 = dot3(t.e20, t.e20);
-    t.rq10 = 1.0f / t.q10;
-    t.rq21 = 1.0f / t.q21;
-    t.rq20 = 1.0f / t.q20;
+    t.rq10 = recip_fixed(t.q10);
+    t.rq21 = recip_fixed(t.q21);
+    t.rq20 = recip_fixed(t.q20);
     return t;
 }
 
