@@ -104,9 +104,6 @@ __device__ __forceinline__ float div_fixed(float a, float b, float y) {
     }
     return q;
 }
-__device__ __forceinline__ f3 divs_fixed(f3 a, float b, float y) {
-    return f3{div_fixed(a.x, b, y), div_fixed(a.y, b, y), div_fixed(a.z, b, y)};
-}
 __device__ __forceinline__ float fmn(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ float fmx(float a, float b) { return fmaxf(a, b); }
 
@@ -326,8 +323,7 @@ struct alignas(16) DevParams {
     int tangent_w_uniform;  // all guide tangent w equal -> tangent_w
     float tangent_w;
     float sdf_inv_cell;
-    int sdf_bx, sdf_by, sdf_bz;
-    float sdf_inv_h;        // recip_fixed(0.1 * cell)
+    int sdf_bx, sdf_by, sdf_bz, pad2;
 };
 
 // Device pointers and counts of one hair object.
@@ -365,9 +361,8 @@ struct SdfGrid {
     // other than 1e10.  Null = no map (every lookup is exact).
     const uint8_t* bricks;
     const uint32_t* brick_bits; // the same occupancy, one bit per brick
-    float inv_cell;             // recip_fixed(cell) (the quotients by cell, the brick test)
+    float inv_cell;             // 1 / cell, for the conservative brick test only
     int bx, by, bz;             // bricks per axis
-    float inv_h;                // recip_fixed(0.1 * cell) (the gradient step)
 };
 
 constexpr int kBrickShift = 2;  // 4^3 cells per brick
@@ -395,9 +390,9 @@ struct MeshSkinJob {
 
 // GetSdfCoordinates :103-113
 __device__ __forceinline__ void sdf_coords(const SdfGrid& g, f3 p, int& cx, int& cy, int& cz) {
-    cx = ftoi(div_fixed(p.x - g.ox, g.cell, g.inv_cell));
-    cy = ftoi(div_fixed(p.y - g.oy, g.cell, g.inv_cell));
-    cz = ftoi(div_fixed(p.z - g.oz, g.cell, g.inv_cell));
+    cx = ftoi((p.x - g.ox) / g.cell);
+    cy = ftoi((p.y - g.oy) / g.cell);
+    cz = ftoi((p.z - g.oz) / g.cell);
 }
 // GetSdfCellIndex :123-129 (wrapping int arithmetic)
 __device__ __forceinline__ int sdf_index(const SdfGrid& g, int x, int y, int z) {
@@ -462,7 +457,7 @@ __device__ __forceinline__ float sdf_distance_t(const SdfGrid& g, const float* _
     for (int j = 0; j < 8; ++j) outside |= (d[j] == INITIAL);
     if (outside) return INITIAL;
     f3 cp = sdf_cell_pos(g, cx, cy, cz);
-    f3 t = divs_fixed(sub(p, cp), g.cell, g.inv_cell);
+    f3 t = divs(sub(p, cp), g.cell);
     float a = lerp_(lerp_(d[0], d[1], t.x), lerp_(d[2], d[3], t.x), t.y);
     float b = lerp_(lerp_(d[4], d[5], t.x), lerp_(d[6], d[7], t.x), t.y);
     return lerp_(a, b, t.z);
@@ -477,7 +472,7 @@ __device__ __forceinline__ bool sdf_project(const SdfGrid& g, const float* __res
     float fx = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 1.0f * h, p.y + 0.0f * h, p.z + 0.0f * h});
     float fy = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 0.0f * h, p.y + 1.0f * h, p.z + 0.0f * h});
     float fz = sdf_distance_t<kAllCorners>(g, sdf, f3{p.x + 0.0f * h, p.y + 0.0f * h, p.z + 1.0f * h});
-    f3 grad = divs_fixed(f3{fx - distance, fy - distance, fz - distance}, h, g.inv_h);
+    f3 grad = divs(f3{fx - distance, fy - distance, fz - distance}, h);
     f3 n = normalize3(grad);
     if (distance < margin) {
         out = add(p, mul(n, margin - distance));

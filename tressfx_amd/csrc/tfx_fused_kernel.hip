@@ -354,7 +354,6 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
         grid.brick_bits = nullptr;
         grid.inv_cell = P.sdf_inv_cell;
         grid.bx = P.sdf_bx; grid.by = P.sdf_by; grid.bz = P.sdf_bz;
-        grid.inv_h = P.sdf_inv_h;
         auto maybe_near = [&](f3 p) -> bool {
             if (!bits_lds) return true;
             int b;

@@ -331,8 +331,7 @@ inline void pack_dev_params(const tfx_hair* h, const tfx_sdf* sdf, DevParams& d)
         d.has_sdf = 1;
         // m_CollisionMargin * m_CellSize (TressFXSDFCollision.cpp:194)
         d.sdf_margin = sdf->margin * sdf->cell;
-        d.sdf_inv_cell = tfx::recip_fixed(sdf->cell);
-        d.sdf_inv_h = tfx::recip_fixed(0.1f * sdf->cell);
+        d.sdf_inv_cell = 1.0f / sdf->cell;
         d.sdf_bx = sdf->bx; d.sdf_by = sdf->by; d.sdf_bz = sdf->bz;
     }
     d.tangent_w_uniform = h->tangent_w_uniform;
@@ -379,9 +378,8 @@ inline SdfGrid sdf_grid(const tfx_sdf* s) {
     g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.pad = 0;
     g.bricks = s->bricks;
     g.brick_bits = s->brick_bits;
-    g.inv_cell = tfx::recip_fixed(s->cell);
+    g.inv_cell = 1.0f / s->cell;
     g.bx = s->bx; g.by = s->by; g.bz = s->bz;
-    g.inv_h = tfx::recip_fixed(0.1f * s->cell);
     return g;
 }
 

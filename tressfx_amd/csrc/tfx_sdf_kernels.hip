@@ -210,14 +210,30 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize(uint32_t* __restrict
     finalize_cells(sdf, ncell, g, blockIdx.x * kSdfBlock + threadIdx.x, gridDim.x * kSdfBlock);
 }
 
+// Word w of the one-bit-per-brick occupancy (bricks 32w..32w+31): a full
+// word's 32 bytes are read as two 16-byte loads.
+__device__ __forceinline__ uint32_t pack_brick_word(const uint8_t* __restrict__ bricks, int nbrick, int w) {
+    uint32_t m = 0;
+    if (w * 32 + 32 <= nbrick) {
+        const uint4* p = reinterpret_cast<const uint4*>(bricks + w * 32);
+        const uint4 a = ldg(p), b = ldg(p + 1);
+        const uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) m |= (((v[k] >> (8 * j)) & 0xffu) != 0 ? 1u : 0u) << (4 * k + j);
+    } else {
+        for (int k = 0; w * 32 + k < nbrick; ++k) m |= (bricks[w * 32 + k] != 0 ? 1u : 0u) << k;
+    }
+    return m;
+}
+
 // Brick occupancy packed to one bit per brick (word w covers bricks 32w..32w+31).
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_bricks(const uint8_t* __restrict__ bricks, int nbrick,
                                                                uint32_t* __restrict__ bits) {
     const int w = blockIdx.x * kSdfBlock + threadIdx.x;
     if (w * 32 >= nbrick) return;
-    uint32_t m = 0;
-    for (int k = 0; k < 32 && w * 32 + k < nbrick; ++k) m |= (bricks[w * 32 + k] != 0 ? 1u : 0u) << k;
-    bits[w] = m;
+    bits[w] = pack_brick_word(bricks, nbrick, w);
 }
 
 // ConstructSignedDistanceField :329-397 for one triangle, spread over the
@@ -380,11 +396,8 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_multi(const SdfJob* __re
     const SdfJob& j = jobs[blockIdx.y];
     const int nbrick = j.g.bx * j.g.by * j.g.bz;
     uint32_t* bits = const_cast<uint32_t*>(j.g.brick_bits);
-    for (int w = blockIdx.x * kSdfBlock + threadIdx.x; w * 32 < nbrick; w += gridDim.x * kSdfBlock) {
-        uint32_t m = 0;
-        for (int k = 0; k < 32 && w * 32 + k < nbrick; ++k) m |= (j.g.bricks[w * 32 + k] != 0 ? 1u : 0u) << k;
-        bits[w] = m;
-    }
+    for (int w = blockIdx.x * kSdfBlock + threadIdx.x; w * 32 < nbrick; w += gridDim.x * kSdfBlock)
+        bits[w] = pack_brick_word(j.g.bricks, nbrick, w);
 }
 
 }  // namespace tfx
