@@ -207,6 +207,58 @@ const char* tfx_last_error(void);
  * "ConstructSignedDistanceField", "FinalizeSignedDistanceField",
  * "CollideHairVerticesWithSdf") and forgets them. */
 int tfx_profiling_enable(int enable);
+
+/* ---- Mesh-based strand transforms ---------------------------------------- */
+/* HairToTriangleMapping and Transforms, TressFXSimulation.hlsl:146-158. */
+typedef struct tfx_hair_to_triangle_mapping {
+    uint32_t mesh_index;
+    uint32_t triangle_index;     /* triangle of the (non-indexed) mesh: vertices 3t, 3t+1, 3t+2 */
+    float barycentric[3];
+    uint32_t reserved;
+} tfx_hair_to_triangle_mapping;
+typedef struct tfx_strand_transform {
+    float tfm[16];               /* row-major float4x4, row-vector convention */
+    float quat[4];               /* x, y, z, w */
+} tfx_strand_transform;
+/* GenerateTransforms (TressFXSimulation.hlsl:1035-1142, built when
+ * USE_MESH_BASED_HAIR_TRANSFORM is 1): for every guide strand g, with
+ * s = g * (F + 1), the transform taking the strand's root from the rest
+ * triangle mapping[s] of `mesh_vertices` to the same triangle of
+ * `skinned_vertices` (float4 per triangle corner, device memory), written to
+ * transforms[s] (device memory, one record per strand). */
+int tfx_hair_generate_transforms(tfx_hair* hair, const tfx_hair_to_triangle_mapping* mapping,
+                                 const float* mesh_vertices, const float* skinned_vertices, int num_mesh_vertices,
+                                 tfx_strand_transform* transforms, void* stream);
+/* (A strand whose triangle lies past num_mesh_vertices is skipped: its record is not written.) */
+
+/* ---- Rendering level of detail (host arithmetic, no device) -------------- */
+/* The LOD fields of TressFXRenderingSettings (TressFXSettings.h:93-128). */
+typedef struct tfx_rendering_lod_settings {
+    float lod_start_distance;           /* m_LODStartDistance (1) */
+    float lod_end_distance;             /* m_LODEndDistance (5) */
+    float lod_percent;                  /* m_LODPercent (0.5) */
+    float lod_width_multiplier;         /* m_LODWidthMultiplier (2) */
+    float shadow_lod_start_distance;    /* m_ShadowLODStartDistance (1) */
+    float shadow_lod_end_distance;      /* m_ShadowLODEndDistance (5) */
+    float shadow_lod_percent;           /* m_ShadowLODPercent (0.5) */
+    float shadow_lod_width_multiplier;  /* m_ShadowLODWidthMultiplier (2) */
+    float fiber_radius;                 /* m_FiberRadius (0.0021) */
+    int enable_hair_lod;                /* m_EnableHairLOD (0); gates the shadow LOD too, as in the reference */
+} tfx_rendering_lod_settings;
+void tfx_rendering_lod_settings_defaults(tfx_rendering_lod_settings* out);
+/* The LOD part of TressFXHairObject::UpdateRenderingParameters
+ * (TressFXHairObject.cpp:461-481): the strand fiber radius and the hair
+ * density (m_LODHairDensity) at a camera distance, for the colour or the
+ * shadow pass. */
+int tfx_rendering_lod(const tfx_rendering_lod_settings* settings, float distance, int shadow_update,
+                      float* fiber_radius, float* hair_density);
+/* The index count of the strand draw at a hair density
+ * (TressFXHairObject.cpp:260-277): total_indices / 3 primitives scaled by
+ * the density, rounded up to whole strands and down to whole groups of a
+ * guide and its follow strands.  total_indices = 6 * strands * (V - 1)
+ * (TressFXAsset::GetNumHairTriangleIndices). */
+int tfx_num_indices_to_render(long long total_indices, int vertices_per_strand, int follow_per_guide,
+                              float hair_density, long long* num_indices);
 int tfx_profiling_read(const char* label, double* total_ms, int* launches);
 int tfx_get_version(int* major, int* minor, int* patch); /* AMD_TressFX.h:33-35 */
 /* Selects the HIP device used by objects created afterwards on this thread. */
@@ -240,6 +292,16 @@ int tfx_hair_get_simulation_params(const tfx_hair* hair, tfx_simulation_params* 
                                    float* bone_matrices_out /* 16*num_bones or NULL */);
 /* GetDynamicState, TressFXHairObject.h:112. */
 int tfx_hair_get_dynamic_state(tfx_hair* hair, tfx_dynamic_state* out);
+/* TressFXDynamicState::TransitionSimToRendering / TransitionRenderingToSim
+ * (TressFXHairObject.cpp:559-583): the resource barriers between the
+ * simulation and the strand renderer.  Here a barrier between two streams:
+ * work enqueued later on `to_stream` waits for everything enqueued so far on
+ * `from_stream` (nothing to do when they are the same stream).  A renderer
+ * then reads the Pos/Tangent buffers of tfx_hair_get_dynamic_state in place:
+ * float4 per vertex, strand-major, follow strands after their guide, vertex
+ * index = triangle index / 2 of the asset's index buffer (TressFXStrands.hlsl:33-35, 82-89). */
+int tfx_hair_transition_sim_to_rendering(tfx_hair* hair, void* from_stream, void* to_stream);
+int tfx_hair_transition_rendering_to_sim(tfx_hair* hair, void* from_stream, void* to_stream);
 /* Snapshot / restore of the dynamic state (host arrays; NULL entries skipped).
  * Both are stream-ordered and synchronise `stream` before returning. */
 int tfx_hair_download_state(tfx_hair* hair, void* stream, float* positions, float* positions_prev,
@@ -296,6 +358,16 @@ int tfx_sdf_set_mesh(tfx_sdf* sdf, const tfx_sdf_mesh_desc* mesh);
 /* Update, TressFXSDFCollision.cpp:95-169: origin from the current bbox, then
  * Initialize -> Construct -> Finalize. */
 int tfx_sdf_update(tfx_sdf* sdf, tfx_sdf_collision_system* sys, void* stream);
+/* Which collision kernel CollideWithHair runs (the compute entry point the
+ * reference's TressFXSDFCollisionSystem binds, TressFXSDFCollision.h:49):
+ * TFX_SDF_COLLIDE_FORWARD, the default, is CollideHairVerticesWithSdf_forward
+ * (TressFXSDFCollision.hlsl:531-595); TFX_SDF_COLLIDE_MIXED is
+ * CollideHairVerticesWithSdf (:597-715), forward/backward differences inside
+ * one cube and no root-vertex test.  The fused and batched paths run the
+ * forward kernel only and refuse a mixed SDF with TFX_ERR_UNSUPPORTED. */
+#define TFX_SDF_COLLIDE_FORWARD 0
+#define TFX_SDF_COLLIDE_MIXED 1
+int tfx_sdf_set_collide_variant(tfx_sdf* sdf, int variant);
 /* CollideWithHair, TressFXSDFCollision.cpp:171-221. */
 int tfx_sdf_collide_with_hair(tfx_sdf* sdf, tfx_sdf_collision_system* sys, void* stream, tfx_hair* hair);
 int tfx_sdf_get_grid(const tfx_sdf* sdf, tfx_sdf_grid_info* out);  /* grid getters, .h:76-93 */

@@ -48,6 +48,8 @@ def load():
     for p in ("32", "64"):
         getattr(lib, f"orc{p}_sdf_build").argtypes = [_VP, _VP, _I, _VP, R[p], _I, _I, _I, _VP, _I]
         getattr(lib, f"orc{p}_sdf_collide").argtypes = [_VP, _VP, _I, _I, _VP, _VP, R[p], _I, _I, _I, R[p], _I]
+        getattr(lib, f"orc{p}_sdf_collide_mixed").argtypes = [_VP, _VP, _I, _VP, _VP, R[p], _I, _I, _I, R[p], _I]
+        getattr(lib, f"orc{p}_generate_transforms").argtypes = [_VP, _I, _I, _I, _VP, _VP, _VP, _I, _VP, _I]
     for p in ("32", "64"):
         getattr(lib, f"orc{p}_mesh_skin").argtypes = [_VP, _VP, _I, _VP, _VP, _I]
     lib.orc32_get_signed_distance.argtypes = [_VP, _VP, _F, _I, _I, _I, _VP]
@@ -139,14 +141,40 @@ def sdf_build(verts, indices, origin, cell, dims, dtype=np.float64, nthreads=0):
     return out
 
 
-def sdf_collide(pos, prev, V, sdf, origin, cell, dims, margin, nthreads=0):
+def sdf_collide(pos, prev, V, sdf, origin, cell, dims, margin, nthreads=0, mixed=False):
+    """CollideHairVerticesWithSdf_forward (or, with mixed=True, the mixed
+    forward/backward-difference CollideHairVerticesWithSdf) in place."""
     lib = load()
     dt = pos.dtype
     o = np.ascontiguousarray(origin, dt)
-    fn = lib.orc64_sdf_collide if dt == np.float64 else lib.orc32_sdf_collide
     n = pos.shape[0]
+    if mixed:
+        fn = lib.orc64_sdf_collide_mixed if dt == np.float64 else lib.orc32_sdf_collide_mixed
+        fn(_p(pos), _p(prev), n, _p(np.ascontiguousarray(sdf, dt)), _p(o), cell, int(dims[0]), int(dims[1]),
+           int(dims[2]), margin, nthreads)
+        return
+    fn = lib.orc64_sdf_collide if dt == np.float64 else lib.orc32_sdf_collide
     fn(_p(pos), _p(prev), n, V, _p(np.ascontiguousarray(sdf, dt)), _p(o), cell, int(dims[0]), int(dims[1]),
        int(dims[2]), margin, nthreads)
+
+
+def generate_transforms(rest, G, F, V, mapping, mesh_vertices, skinned_vertices, dtype=np.float32, nthreads=0):
+    """GenerateTransforms transcription: rest float[S*V, 4] (initial hair
+    positions), mapping: a ctypes array or numpy buffer of
+    tfx_hair_to_triangle_mapping[S], mesh/skinned float[3*T, 4] ->
+    float[S, 20] (row-major 4x4, quaternion); rows of follow strands, and of
+    strands whose triangle lies past the mesh, stay zero."""
+    lib = load()
+    dt = np.dtype(dtype)
+    r = np.ascontiguousarray(rest, dt)
+    m = np.ascontiguousarray(mesh_vertices, dt)
+    k = np.ascontiguousarray(skinned_vertices, dt)
+    mp = np.ascontiguousarray(mapping)
+    S = G * (F + 1)
+    out = np.zeros((S, 20), dt)
+    fn = lib.orc64_generate_transforms if dt == np.float64 else lib.orc32_generate_transforms
+    fn(_p(r), G, F, V, mp.ctypes.data, _p(m), _p(k), m.shape[0], _p(out), nthreads)
+    return out
 
 
 def mesh_skin(rest, skin, bones, dtype=np.float32, nthreads=0):

@@ -711,6 +711,107 @@ template <class R> void sdf_collide(R* pos, R* prev, int N, int V, const R* sdf,
     });
 }
 
+// CollideHairVerticesWithSdf :597-715, the variant mixing forward and
+// backward differences (compiled by the reference, not dispatched): every
+// vertex, the gradient from the same eight nodes as the distance, h = 0.1 *
+// cell added to the cell-unit interpolation weights as written.
+template <class R> void sdf_collide_mixed(R* pos, R* prev, int N, const R* sdf, const Grid<R>& g, R margin,
+                                          int nthreads) {
+    const R INITIAL = R(1e10f);
+    parallel_for(N, nthreads, [&](int i) {
+        V4<R> hv = load4(pos, (size_t)i);
+        V3<R> p = xyz(hv);
+        int c[3];
+        sdf_coords(g, p, c);
+        if (!(0 <= c[0] && c[0] < g.nx - 2) || !(0 <= c[1] && c[1] < g.ny - 2) || !(0 <= c[2] && c[2] < g.nz - 2))
+            return;
+        int base = sdf_index(g, c[0], c[1], c[2]);
+        int X = 1, Y = g.nx, Z = g.ny * g.nx;
+        int ids[8] = {base, base + X, base + Y, base + Y + X, base + Z, base + Z + X, base + Z + Y, base + Z + Y + X};
+        R d[8];
+        for (int j = 0; j < 8; ++j) {
+            d[j] = sdf[ids[j]];
+            if (d[j] == INITIAL) return;
+        }
+        auto tri = [&](R x, R y, R z) {
+            return lerp_(bilerp(d[0], d[1], d[2], d[3], x, y), bilerp(d[4], d[5], d[6], d[7], x, y), z);
+        };
+        V3<R> cp = sdf_cell_pos(g, c[0], c[1], c[2]);
+        V3<R> t = (p - cp) / g.cell;
+        R dv = tri(t.x, t.y, t.z);
+        R h = R(0.1f) * g.cell;
+        V3<R> dir{(t.x + h < R(1)) ? R(1) : R(-1), (t.y + h < R(1)) ? R(1) : R(-1), (t.z + h < R(1)) ? R(1) : R(-1)};
+        V3<R> nb{tri(t.x + h * dir.x, t.y, t.z), tri(t.x, t.y + h * dir.y, t.z), tri(t.x, t.y, t.z + h * dir.z)};
+        V3<R> grad = V3<R>{dir.x * (nb.x - dv), dir.y * (nb.y - dv), dir.z * (nb.z - dv)} / h;
+        V3<R> n = normalize(grad);
+        if (dv < margin) {
+            V3<R> proj = p + n * (margin - dv);
+            store3(pos, (size_t)i, proj);
+            store3(prev, (size_t)i, proj);
+        }
+    });
+}
+
+// GenerateTransforms, TressFXSimulation.hlsl:1035-1142.  Row-major 4x4
+// products summed left to right, ((a0 b0 + a1 b1) + a2 b2) + a3 b3.
+template <class R> struct M4 { R m[4][4]; };
+template <class R> M4<R> m4_mul(const M4<R>& a, const M4<R>& b) {
+    M4<R> c;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            c.m[i][j] = ((a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j]) + a.m[i][3] * b.m[3][j];
+    return c;
+}
+template <class R> M4<R> m4_frame(V3<R> t, V3<R> n, V3<R> b) {
+    M4<R> m{};
+    m.m[0][0] = t.x; m.m[0][1] = t.y; m.m[0][2] = t.z;
+    m.m[1][0] = n.x; m.m[1][1] = n.y; m.m[1][2] = n.z;
+    m.m[2][0] = b.x; m.m[2][1] = b.y; m.m[2][2] = b.z;
+    m.m[3][3] = R(1);
+    return m;
+}
+template <class R> M4<R> m4_triangle_frame(V3<R> v1, V3<R> v2, V3<R> v3) {
+    V3<R> tangent = normalize(v1 - v3);
+    V3<R> tangent2 = v2 - v3;
+    V3<R> normal = normalize(cross(tangent, tangent2));
+    V3<R> binormal = normalize(cross(normal, tangent));
+    return m4_frame(tangent, normal, binormal);
+}
+template <class R> void generate_transforms(const R* rest, int G, int F, int V, const tfx_hair_to_triangle_mapping* map,
+                                            const R* mesh, const R* skinned, int nverts, R* out, int nthreads) {
+    parallel_for(G, nthreads, [&](int g) {
+        const size_t s = (size_t)g * (F + 1);
+        const size_t tri = (size_t)map[s].triangle_index * 3;
+        if ((long long)tri + 2 >= (long long)nverts) return;  // a triangle past the mesh: record not written
+        const R a = map[s].barycentric[0], b = map[s].barycentric[1], c = map[s].barycentric[2];
+        V3<R> v1 = xyz(load4(mesh, tri)), v2 = xyz(load4(mesh, tri + 1)), v3 = xyz(load4(mesh, tri + 2));
+        V3<R> s1 = xyz(load4(skinned, tri)), s2 = xyz(load4(skinned, tri + 1)), s3 = xyz(load4(skinned, tri + 2));
+        V3<R> tfmPos = (s1 * a + s2 * b) + s3 * c;
+        V3<R> initialPos = xyz(load4(rest, s * V));
+        M4<R> triM = m4_triangle_frame(v1, v2, v3);
+        M4<R> tfmTriM = m4_triangle_frame(s1, s2, s3);
+        M4<R> triT{};
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) triT.m[i][j] = triM.m[j][i];
+        M4<R> rot = m4_mul(triT, tfmTriM);
+        M4<R> tr{};
+        for (int i = 0; i < 4; ++i) tr.m[i][i] = R(1);
+        tr.m[3][0] = -initialPos.x; tr.m[3][1] = -initialPos.y; tr.m[3][2] = -initialPos.z;
+        rot = m4_mul(tr, rot);
+        tr.m[3][0] = tfmPos.x; tr.m[3][1] = tfmPos.y; tr.m[3][2] = tfmPos.z;
+        M4<R> tfm = m4_mul(rot, tr);
+        R q[4];
+        q[3] = std::sqrt(((R(1) + tfm.m[0][0]) + tfm.m[1][1]) + tfm.m[2][2]) / R(2);
+        q[0] = (tfm.m[2][1] - tfm.m[1][2]) / (R(4) * q[3]);
+        q[1] = (tfm.m[0][2] - tfm.m[2][0]) / (R(4) * q[3]);
+        q[2] = (tfm.m[1][0] - tfm.m[0][1]) / (R(4) * q[3]);
+        R* o = out + s * 20;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) o[4 * i + j] = tfm.m[i][j];
+        for (int k = 0; k < 4; ++k) o[16 + k] = q[k];
+    });
+}
+
 // ---------------------------------------------------------------------------
 // Host parameter packing, TressFXHairObject.cpp:172-235 and :380-437
 // (Quaternion.cpp :79-89 SetRotation(axis, angle), :262-284 products).
@@ -841,6 +942,24 @@ int orc64_sdf_collide(double* pos, double* prev, int N, int V, const double* sdf
                       int nx, int ny, int nz, double margin, int nthreads) {
     Grid<double> g{V3<double>{origin[0], origin[1], origin[2]}, cell, nx, ny, nz};
     sdf_collide<double>(pos, prev, N, V, sdf, g, margin, nthreads); return 0;
+}
+int orc32_sdf_collide_mixed(float* pos, float* prev, int N, const float* sdf, const float* origin, float cell,
+                            int nx, int ny, int nz, float margin, int nthreads) {
+    Grid<float> g{V3<float>{origin[0], origin[1], origin[2]}, cell, nx, ny, nz};
+    sdf_collide_mixed<float>(pos, prev, N, sdf, g, margin, nthreads); return 0;
+}
+int orc64_sdf_collide_mixed(double* pos, double* prev, int N, const double* sdf, const double* origin, double cell,
+                            int nx, int ny, int nz, double margin, int nthreads) {
+    Grid<double> g{V3<double>{origin[0], origin[1], origin[2]}, cell, nx, ny, nz};
+    sdf_collide_mixed<double>(pos, prev, N, sdf, g, margin, nthreads); return 0;
+}
+int orc32_generate_transforms(const float* rest, int G, int F, int V, const tfx_hair_to_triangle_mapping* map,
+                              const float* mesh, const float* skinned, int nverts, float* out, int nthreads) {
+    generate_transforms<float>(rest, G, F, V, map, mesh, skinned, nverts, out, nthreads); return 0;
+}
+int orc64_generate_transforms(const double* rest, int G, int F, int V, const tfx_hair_to_triangle_mapping* map,
+                              const double* mesh, const double* skinned, int nverts, double* out, int nthreads) {
+    generate_transforms<double>(rest, G, F, V, map, mesh, skinned, nverts, out, nthreads); return 0;
 }
 float orc32_get_signed_distance(const float* sdf, const float* origin, float cell, int nx, int ny, int nz, const float* p) {
     Grid<float> g{V3<float>{origin[0], origin[1], origin[2]}, cell, nx, ny, nz};

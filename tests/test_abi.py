@@ -54,6 +54,9 @@ RECORDS = {
     "tfx_server_config": tfx.ServerConfig,
     "tfx_instance_desc": tfx.InstanceDesc,
     "tfx_server_stats": tfx.ServerStats,
+    "tfx_rendering_lod_settings": tfx.RenderingLodSettings,
+    "tfx_hair_to_triangle_mapping": tfx.HairToTriangleMapping,
+    "tfx_strand_transform": tfx.StrandTransform,
 }
 
 

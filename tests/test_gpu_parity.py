@@ -247,6 +247,58 @@ def test_collide_and_fused_vs_oracle(gpu):
     assert hits > 0, "the SDF never moved a vertex: the test does not exercise the collision"
 
 
+def test_collide_mixed_difference_variant_vs_oracle(gpu):
+    """CollideWithHair with the mixed forward/backward-difference kernel
+    (TressFXSDFCollision.hlsl:597-715, every vertex including the roots),
+    stepwise against the FP32 transcription, bit for bit; the fused and
+    batched paths refuse a mixed SDF (TFX_ERR_UNSUPPORTED)."""
+    torch = gpu
+    print("[collide mixed] plain vs oracle", flush=True)
+    wl, dm = _mesh_setup(torch, seed=23)
+    origin, cell, dims = wl.sdf_grid
+    system = tfx.TressFXSDFCollisionSystem()
+    sim = tfx.TressFXSimulation()
+    hair = tfx.TressFXHairObject(wl.asset)
+    sdf = tfx.TressFXSDFCollision(dm.update(0.0), collision_margin=wl.collision_margin, explicit=(origin, cell, dims))
+    sdf.SetCollideVariant(tfx.TFX_SDF_COLLIDE_MIXED)
+    o32 = oracle.OracleHair(wl.asset, np.float32)
+    V = wl.asset.num_vertices_per_strand
+    hits = roots = 0
+    for frame in range(4):
+        sdf.SetMesh(dm.update(frame / synth.FRAME_RATE))
+        sdf.Update(system)
+        grid32 = sdf.download()
+        for sub in range(wl.substeps):
+            set_substep_inputs(hair, wl, frame, sub)
+            params, bones = hair.simulation_params()
+            o32.set_state(hair.download_state())
+            cp = sdf.collision_params(hair)
+            sim.Simulate([hair])
+            sdf.CollideWithHair(system, hair)
+            o32.simulate(params, bones)
+            before = o32.pos.copy()
+            oracle.sdf_collide(o32.pos, o32.prev, V, grid32, np.array(cp.origin[:3], np.float32),
+                               np.float32(cp.cell_size), dims, np.float32(cp.collision_margin), mixed=True)
+            moved = (before != o32.pos).any(axis=1)
+            hits += int(moved.sum())
+            roots += int(moved.reshape(-1, V)[:, :2].sum())
+            compare_state_fp32(hair.download_state(), o32, f"mixed frame {frame} sub {sub}")
+    print(f"mixed collide hits {hits} (root vertices {roots})")
+    assert hits > 0
+    with pytest.raises(tfx.TFXError) as e:
+        sim.SimulateFused([hair], [sdf])
+    assert e.value.code == 6
+    b = tfx.HairBatch()
+    b.add(tfx.TressFXHairObject(wl.asset), sdf)
+    with pytest.raises(tfx.TFXError) as e:
+        b.begin_frame()
+    assert e.value.code == 6
+    sdf.SetCollideVariant(tfx.TFX_SDF_COLLIDE_FORWARD)
+    sim.SimulateFused([hair], [sdf])
+    with pytest.raises(tfx.TFXError):
+        sdf.SetCollideVariant(7)
+
+
 @pytest.mark.parametrize("variant", [-1, 0, 1, 2])
 def test_fused_multi_object_mixed_lengths(gpu, variant):
     """Several objects of different strand lengths in one fused call (one

@@ -598,6 +598,112 @@ def test_collide_plane_sdf_exact():
     assert np.array_equal(pos2, p2)
 
 
+def test_collide_mixed_differences_plane_exact():
+    """The mixed forward/backward-difference variant (TressFXSDFCollision.hlsl
+    :597-715) on a linear field: every vertex -- strand roots included, as
+    that kernel has no root test -- at depth d below the plane moves to
+    +margin along the normal, whether each axis takes the forward or, near the
+    cell's far face, the backward difference (a sign slip in either would turn
+    the normal); the step h = 0.1 * cell is added to cell-unit weights, which
+    only scales the gradient.  A cube holding a 1e10 node and vertices outside
+    the grid stay put."""
+    n, cell = 16, 0.01
+    origin = np.zeros(3)
+    ii = np.indices((n, n, n)).reshape(3, -1)[::-1].T
+    pts = ii * cell + origin
+    nrm = np.array([0.0, 0.6, 0.8])
+    z0 = 0.07
+    sdf = pts @ nrm - z0
+    # interpolation weights: x = 0.5 (forward), x = 0.9995 (backward), y/z likewise
+    pos = np.array([[0.055, 0.055, 0.045, 0.0], [0.059995, 0.059995, 0.04, 1.0], [0.0455, 0.04, 0.059995, 1.0],
+                    [0.06, 0.06, 0.07, 1.0], [0.155, 0.155, 0.155, 1.0], [-0.5, 0.02, 0.02, 1.0]])
+    h = 0.1 * cell
+    frac = (pos[:, :3] - origin) / cell - np.floor((pos[:, :3] - origin) / cell)
+    assert np.all(frac[0] + h < 1.0)                                     # forward in every axis
+    assert frac[1, 0] + h >= 1.0 and frac[1, 1] + h >= 1.0 and frac[1, 2] + h < 1.0   # backward in x and y
+    assert frac[2, 2] + h >= 1.0                                         # backward in z
+    prev = pos.copy()
+    before = pos.copy()
+    margin = 0.005
+    oracle.sdf_collide(pos, prev, 4, sdf, origin, cell, (n, n, n), margin, mixed=True)
+    for i in range(4):
+        d = before[i, :3] @ nrm - z0
+        if d >= margin:
+            assert np.array_equal(pos[i], before[i]), i
+        else:
+            exp = before[i, :3] + nrm * (margin - d)
+            assert np.allclose(pos[i, :3], exp, atol=1e-12), i
+            assert np.array_equal(prev[i, :3], pos[i, :3]) and prev[i, 3] == before[i, 3], i
+    moved = [i for i in range(4) if not np.array_equal(pos[i], before[i])]
+    assert 0 in moved and 1 in moved and 2 in moved                       # a root (index 0) moves too
+    assert np.array_equal(pos[4:], before[4:])                            # beyond n - 2 cells, outside
+    # the forward variant leaves that root alone
+    f = before.copy()
+    oracle.sdf_collide(f, f.copy(), 4, sdf, origin, cell, (n, n, n), margin)
+    assert np.array_equal(f[0], before[0]) and not np.array_equal(f[2], before[2])
+    # a cube with a 1e10 node is skipped
+    sdf2 = sdf.copy()
+    c = (np.floor((before[1, :3] - origin) / cell)).astype(int)
+    sdf2[(c[2] * n + c[1]) * n + c[0] + 1] = 1e10
+    p2 = before.copy()
+    oracle.sdf_collide(p2, p2.copy(), 4, sdf2, origin, cell, (n, n, n), margin, mixed=True)
+    assert np.array_equal(p2[1], before[1])
+
+
+def _transform_case(rng, G, F, V, T):
+    """A non-indexed mesh of T triangles, a mapping of every guide to a
+    random triangle and barycentric point, and rest strands rooted there."""
+    mesh = np.ones((3 * T, 4))
+    mesh[:, :3] = rng.normal(size=(3 * T, 3))
+    S = G * (F + 1)
+    mapping = (tfx.HairToTriangleMapping * S)()
+    rest = np.zeros((S * V, 4))
+    rest[:, 3] = 1.0
+    for g in range(G):
+        s = g * (F + 1)
+        t = int(rng.integers(0, T))
+        b = rng.dirichlet([1.0, 1.0, 1.0]).astype(np.float32)
+        mapping[s].triangle_index = t
+        mapping[s].barycentric[0], mapping[s].barycentric[1], mapping[s].barycentric[2] = (float(x) for x in b)
+        tri = mesh[3 * t:3 * t + 3, :3]
+        rest[s * V, :3] = b.astype(np.float64) @ tri
+    return mesh, mapping, rest
+
+
+def test_generate_transforms_closed_forms():
+    """GenerateTransforms (TressFXSimulation.hlsl:1035-1142) in FP64: an
+    unmoved mesh gives the identity transform and quaternion (0, 0, 0, 1);
+    the mesh turned by theta about z (row-vector p R) and moved by T gives R
+    as the rotation block, maps each root to its point on the moved triangle,
+    and the quaternion (0, 0, -sin(theta/2), cos(theta/2)) of the diagonal
+    formula; follow strands and triangles past the mesh keep zero records."""
+    rng = np.random.Generator(np.random.PCG64(41))
+    G, F, V, T = 20, 1, 4, 7
+    mesh, mapping, rest = _transform_case(rng, G, F, V, T)
+    out = oracle.generate_transforms(rest, G, F, V, mapping, mesh, mesh, np.float64)
+    guides = np.arange(G) * (F + 1)
+    for s in guides:
+        m = out[s, :16].reshape(4, 4)
+        assert np.allclose(m, np.eye(4), atol=1e-12), s
+        assert np.allclose(out[s, 16:], [0, 0, 0, 1], atol=1e-12)
+    assert np.all(out[np.setdiff1d(np.arange(G * (F + 1)), guides)] == 0)
+    th = 0.7
+    R = np.array([[np.cos(th), np.sin(th), 0], [-np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    Tr = np.array([0.3, -1.2, 0.5])
+    moved = mesh.copy()
+    moved[:, :3] = mesh[:, :3] @ R + Tr
+    out = oracle.generate_transforms(rest, G, F, V, mapping, mesh, moved, np.float64)
+    for s in guides:
+        m = out[s, :16].reshape(4, 4)
+        assert np.allclose(m[:3, :3], R, atol=1e-12), s
+        root = np.append(rest[s * V, :3], 1.0)
+        assert np.allclose((root @ m)[:3], rest[s * V, :3] @ R + Tr, atol=1e-12), s
+        assert np.allclose(out[s, 16:], [0, 0, -np.sin(th / 2), np.cos(th / 2)], atol=1e-12), s
+    mapping[0].triangle_index = T          # past the mesh
+    out = oracle.generate_transforms(rest, G, F, V, mapping, mesh, moved, np.float64)
+    assert np.all(out[0] == 0)
+
+
 # ---------------------------------------------------------------------------
 # Host parameter packing (UpdateSimulationParameters / SetWind)
 # ---------------------------------------------------------------------------

@@ -222,6 +222,40 @@ class InstanceDesc(ctypes.Structure):
     ]
 
 
+class HairToTriangleMapping(ctypes.Structure):
+    """tfx_hair_to_triangle_mapping (HairToTriangleMapping, TressFXSimulation.hlsl:152-158)."""
+    _fields_ = [("mesh_index", ctypes.c_uint32), ("triangle_index", ctypes.c_uint32),
+                ("barycentric", ctypes.c_float * 3), ("reserved", ctypes.c_uint32)]
+
+
+class StrandTransform(ctypes.Structure):
+    """tfx_strand_transform (Transforms, TressFXSimulation.hlsl:146-150)."""
+    _fields_ = [("tfm", ctypes.c_float * 16), ("quat", ctypes.c_float * 4)]
+
+
+class RenderingLodSettings(ctypes.Structure):
+    """tfx_rendering_lod_settings: the LOD fields of TressFXRenderingSettings
+    (TressFXSettings.h:93-128)."""
+    _fields_ = [
+        ("lod_start_distance", ctypes.c_float),
+        ("lod_end_distance", ctypes.c_float),
+        ("lod_percent", ctypes.c_float),
+        ("lod_width_multiplier", ctypes.c_float),
+        ("shadow_lod_start_distance", ctypes.c_float),
+        ("shadow_lod_end_distance", ctypes.c_float),
+        ("shadow_lod_percent", ctypes.c_float),
+        ("shadow_lod_width_multiplier", ctypes.c_float),
+        ("fiber_radius", ctypes.c_float),
+        ("enable_hair_lod", ctypes.c_int),
+    ]
+
+    @classmethod
+    def defaults(cls):
+        s = cls()
+        load_library().tfx_rendering_lod_settings_defaults(ctypes.byref(s))
+        return s
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [
         ("completed_requests", ctypes.c_longlong),
@@ -244,6 +278,9 @@ class ServerStats(ctypes.Structure):
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _F = ctypes.c_float
+TFX_SDF_COLLIDE_FORWARD = 0
+TFX_SDF_COLLIDE_MIXED = 1
+
 EXPORTS = {
     "tfx_last_error": (ctypes.c_char_p, []),
     "tfx_get_version": (_I, [ctypes.POINTER(_I)] * 3),
@@ -261,6 +298,14 @@ EXPORTS = {
     "tfx_hair_get_info": (_I, [_VP, ctypes.POINTER(HairInfo)]),
     "tfx_hair_get_simulation_params": (_I, [_VP, ctypes.POINTER(SimulationParams), c_float_p]),
     "tfx_hair_get_dynamic_state": (_I, [_VP, ctypes.POINTER(DynamicState)]),
+    "tfx_sdf_set_collide_variant": (_I, [_VP, _I]),
+    "tfx_hair_generate_transforms": (_I, [_VP, _VP, _VP, _VP, _I, _VP, _VP]),
+    "tfx_hair_transition_sim_to_rendering": (_I, [_VP, _VP, _VP]),
+    "tfx_hair_transition_rendering_to_sim": (_I, [_VP, _VP, _VP]),
+    "tfx_rendering_lod_settings_defaults": (None, [ctypes.POINTER(RenderingLodSettings)]),
+    "tfx_rendering_lod": (_I, [ctypes.POINTER(RenderingLodSettings), ctypes.c_float, _I, ctypes.POINTER(ctypes.c_float),
+                               ctypes.POINTER(ctypes.c_float)]),
+    "tfx_num_indices_to_render": (_I, [ctypes.c_longlong, _I, _I, ctypes.c_float, ctypes.POINTER(ctypes.c_longlong)]),
     "tfx_hair_download_state": (_I, [_VP, _VP, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p]),
     "tfx_hair_upload_state": (_I, [_VP, _VP, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p]),
     "tfx_simulation_create": (_I, [ctypes.POINTER(_VP)]),
@@ -361,6 +406,24 @@ def init(device=0):
 
 def profiling_enable(on=True):
     _check(load_library().tfx_profiling_enable(1 if on else 0))
+
+
+def rendering_lod(settings, distance, shadow_update=False):
+    """(fiber radius, hair density) at a camera distance (the LOD part of
+    UpdateRenderingParameters, TressFXHairObject.cpp:461-481)."""
+    r, d = ctypes.c_float(), ctypes.c_float()
+    _check(load_library().tfx_rendering_lod(ctypes.byref(settings), ctypes.c_float(distance), 1 if shadow_update else 0,
+                                            ctypes.byref(r), ctypes.byref(d)))
+    return r.value, d.value
+
+
+def num_indices_to_render(total_indices, vertices_per_strand, follow_per_guide, hair_density):
+    """Index count of the strand draw at a hair density (TressFXHairObject.cpp:260-277)."""
+    out = ctypes.c_longlong()
+    _check(load_library().tfx_num_indices_to_render(int(total_indices), int(vertices_per_strand),
+                                                    int(follow_per_guide), ctypes.c_float(hair_density),
+                                                    ctypes.byref(out)))
+    return out.value
 
 
 def profiling_read(label):
@@ -490,6 +553,21 @@ class TressFXHairObject:
         _check(self._lib.tfx_hair_get_dynamic_state(self.handle, ctypes.byref(o)))
         return o
 
+    def GenerateTransforms(self, mapping_ptr, mesh_vertices_ptr, skinned_vertices_ptr, num_mesh_vertices,
+                           transforms_ptr, stream=None):
+        """GenerateTransforms (TressFXSimulation.hlsl:1035-1142) on device buffers."""
+        _check(self._lib.tfx_hair_generate_transforms(self.handle, mapping_ptr, mesh_vertices_ptr,
+                                                       skinned_vertices_ptr, int(num_mesh_vertices),
+                                                       transforms_ptr, stream))
+
+    def TransitionSimToRendering(self, from_stream, to_stream):
+        """Work enqueued later on to_stream waits for from_stream (TressFXHairObject.cpp:559-569)."""
+        _check(self._lib.tfx_hair_transition_sim_to_rendering(self.handle, from_stream, to_stream))
+
+    def TransitionRenderingToSim(self, from_stream, to_stream):
+        """The barrier back (TressFXHairObject.cpp:571-583)."""
+        _check(self._lib.tfx_hair_transition_rendering_to_sim(self.handle, from_stream, to_stream))
+
     def download_state(self, stream=None):
         n = self.GetNumTotalHairVertices()
         s = self.GetNumTotalHairStrands()
@@ -599,6 +677,10 @@ class TressFXSDFCollision:
 
     def Update(self, system, stream=None):
         _check(self._lib.tfx_sdf_update(self.handle, system.handle, stream))
+
+    def SetCollideVariant(self, variant):
+        """TFX_SDF_COLLIDE_FORWARD (default) or TFX_SDF_COLLIDE_MIXED."""
+        _check(self._lib.tfx_sdf_set_collide_variant(self.handle, int(variant)))
 
     def CollideWithHair(self, system, hair, stream=None):
         _check(self._lib.tfx_sdf_collide_with_hair(self.handle, system.handle, stream, hair.handle))

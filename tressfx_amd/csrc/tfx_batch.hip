@@ -220,6 +220,10 @@ extern "C" int tfx_batch_set_launch_shape(tfx_batch* b, int V, int variant, int 
 extern "C" int tfx_batch_begin_frame(tfx_batch* b) {
     CHECK_ARG(b, "tfx_batch_begin_frame: null batch");
     CHECK_ARG(b->slot < 0, "tfx_batch_begin_frame: previous frame not run");
+    for (size_t m = 0; m < b->members.size(); ++m)
+        if (b->members[m].sdf && b->members[m].sdf->collide_variant != TFX_SDF_COLLIDE_FORWARD)
+            return fail(TFX_ERR_UNSUPPORTED, "tfx_batch_begin_frame: the batched form collides with forward "
+                                             "differences only (member %d has a mixed-variant SDF)", (int)m);
     int rc = prepare(b);
     if (rc) return rc;
     const int k = b->next_slot;

@@ -459,6 +459,9 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
     hipStream_t s = (hipStream_t)stream;
     for (int i = 0; i < n; ++i) {
         CHECK_ARG(hairs[i], "tfx_simulation_simulate_fused: null hair");
+        if (sdfs && sdfs[i] && sdfs[i]->collide_variant != TFX_SDF_COLLIDE_FORWARD)
+            return fail(TFX_ERR_UNSUPPORTED, "tfx_simulation_simulate_fused: the fused substep collides with "
+                                             "forward differences only (SDF %d has the mixed variant)", i);
         int rc = upload_params(hairs[i], sdfs ? sdfs[i] : nullptr, s);
         if (rc) return rc;
     }
@@ -635,8 +638,24 @@ extern "C" int tfx_sdf_collide_with_hair(tfx_sdf* s, tfx_sdf_collision_system* s
     CHECK_ARG(s && sys && h, "tfx_sdf_collide_with_hair: null argument");
     if (!s->mesh.vertices || !s->mesh.indices) return TFX_OK;
     PassTimer tm("CollideHairVerticesWithSdf", (hipStream_t)stream);
-    HIP_TRY(tfx_launch_sdf_collide(h->pos, h->prev, h->N, h->V, reinterpret_cast<const float*>(s->cells), sdf_grid(s),
-                                   s->margin * s->cell, (hipStream_t)stream));
+    const float* cells = reinterpret_cast<const float*>(s->cells);
+    if (s->collide_variant == TFX_SDF_COLLIDE_MIXED)
+        HIP_TRY(tfx_launch_sdf_collide_mixed(h->pos, h->prev, h->N, cells, sdf_grid(s), s->margin * s->cell,
+                                             (hipStream_t)stream));
+    else
+        HIP_TRY(tfx_launch_sdf_collide(h->pos, h->prev, h->N, h->V, cells, sdf_grid(s), s->margin * s->cell,
+                                       (hipStream_t)stream));
+    return TFX_OK;
+}
+
+// The compute entry point TressFXSDFCollisionSystem binds for CollideWithHair
+// (TressFXSDFCollision.h:49 names CollideHairVerticesWithSdf_forward; the
+// shader also holds CollideHairVerticesWithSdf, :597-715).
+extern "C" int tfx_sdf_set_collide_variant(tfx_sdf* s, int variant) {
+    CHECK_ARG(s, "tfx_sdf_set_collide_variant: null sdf");
+    CHECK_ARG(variant == TFX_SDF_COLLIDE_FORWARD || variant == TFX_SDF_COLLIDE_MIXED,
+              "tfx_sdf_set_collide_variant: unknown variant");
+    s->collide_variant = variant;
     return TFX_OK;
 }
 
@@ -676,5 +695,96 @@ extern "C" int tfx_sdf_get_collision_params(const tfx_sdf* s, const tfx_hair* h,
     o->collision_margin = s->margin * s->cell;
     o->num_total_hair_vertices = h->N;
     o->num_hair_vertices_per_strand = h->V;
+    return TFX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rendering hand-off and level of detail
+// ---------------------------------------------------------------------------
+namespace {
+
+int stream_barrier(tfx_hair* h, void* from, void* to, const char* what) {
+    if (!h) return fail(TFX_ERR_INVALID_ARGUMENT, "%s: null hair", what);
+    if (from == to) return TFX_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    if (!h->handoff) HIP_TRY(hipEventCreateWithFlags(&h->handoff, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(h->handoff, (hipStream_t)from));
+    HIP_TRY(hipStreamWaitEvent((hipStream_t)to, h->handoff, 0));
+    return TFX_OK;
+}
+
+}  // namespace
+
+// TressFXDynamicState::TransitionSimToRendering, TressFXHairObject.cpp:559-569.
+extern "C" int tfx_hair_transition_sim_to_rendering(tfx_hair* h, void* from_stream, void* to_stream) {
+    return stream_barrier(h, from_stream, to_stream, "tfx_hair_transition_sim_to_rendering");
+}
+
+// TressFXDynamicState::TransitionRenderingToSim, TressFXHairObject.cpp:571-583.
+extern "C" int tfx_hair_transition_rendering_to_sim(tfx_hair* h, void* from_stream, void* to_stream) {
+    return stream_barrier(h, from_stream, to_stream, "tfx_hair_transition_rendering_to_sim");
+}
+
+// TressFXRenderingSettings defaults, TressFXSettings.h:95-128.
+extern "C" void tfx_rendering_lod_settings_defaults(tfx_rendering_lod_settings* o) {
+    if (!o) return;
+    o->lod_start_distance = 1.0f;
+    o->lod_end_distance = 5.0f;
+    o->lod_percent = 0.5f;
+    o->lod_width_multiplier = 2.0f;
+    o->shadow_lod_start_distance = 1.0f;
+    o->shadow_lod_end_distance = 5.0f;
+    o->shadow_lod_percent = 0.5f;
+    o->shadow_lod_width_multiplier = 2.0f;
+    o->fiber_radius = 0.0021f;
+    o->enable_hair_lod = 0;
+}
+
+// TressFXHairObject::UpdateRenderingParameters, the LOD part, :461-481.
+extern "C" int tfx_rendering_lod(const tfx_rendering_lod_settings* s, float distance, int shadow_update,
+                                 float* fiber_radius, float* hair_density) {
+    CHECK_ARG(s && fiber_radius && hair_density, "tfx_rendering_lod: null argument");
+    float density = 1.0f;
+    float radius = s->fiber_radius;
+    if (s->enable_hair_lod) {
+        const float a = shadow_update ? s->shadow_lod_start_distance : s->lod_start_distance;
+        const float b = shadow_update ? s->shadow_lod_end_distance : s->lod_end_distance;
+        const float minD = std::min(a, b), maxD = std::max(a, b);
+        if (distance > minD) {
+            const float ratio = std::min((distance - minD) / std::max(maxD - minD, 0.00001f), 1.0f);
+            const float maxR = radius * (shadow_update ? s->shadow_lod_width_multiplier : s->lod_width_multiplier);
+            radius = radius + ratio * (maxR - radius);
+            density = 1.0f + ratio * ((shadow_update ? s->shadow_lod_percent : s->lod_percent) - 1.0f);
+        }
+    }
+    *fiber_radius = radius;
+    *hair_density = density;
+    return TFX_OK;
+}
+
+// The draw size of TressFXHairObject::DrawStrands, :260-277, in its uint32
+// arithmetic.
+extern "C" int tfx_num_indices_to_render(long long total_indices, int V, int F, float density, long long* out) {
+    CHECK_ARG(out && total_indices >= 0 && total_indices <= 0xffffffffll && V >= 2 && F >= 0,
+              "tfx_num_indices_to_render: bad argument");
+    uint32_t prims = (uint32_t)total_indices / 3u;
+    if (density != 1.0f) {
+        prims = (uint32_t)((float)prims * density);
+        const uint32_t per_strand = (uint32_t)(V - 1) * 2u;
+        const uint32_t rem = prims % per_strand;
+        prims = rem > 0 ? prims + per_strand - rem : prims;
+        prims = prims - (prims % (per_strand * (uint32_t)(F + 1)));
+    }
+    *out = (long long)prims * 3;
+    return TFX_OK;
+}
+
+// GenerateTransforms, TressFXSimulation.hlsl:1035-1142.
+extern "C" int tfx_hair_generate_transforms(tfx_hair* h, const tfx_hair_to_triangle_mapping* map, const float* mesh,
+                                            const float* skinned, int nverts, tfx_strand_transform* out, void* stream) {
+    CHECK_ARG(h && map && mesh && skinned && out && nverts >= 0, "tfx_hair_generate_transforms: bad argument");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(tfx_launch_generate_transforms(h->rest, h->G, h->F, h->V, map, reinterpret_cast<const float4*>(mesh),
+                                           reinterpret_cast<const float4*>(skinned), nverts, out, (hipStream_t)stream));
     return TFX_OK;
 }

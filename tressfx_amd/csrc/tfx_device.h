@@ -509,6 +509,46 @@ __device__ __forceinline__ void sdf_collide_batch(const SdfGrid& g, const float*
     }
 }
 
+// CollideHairVerticesWithSdf :597-715, the variant mixing forward and
+// backward differences (the reference builds it but dispatches the forward
+// one): the gradient comes from the eight nodes of the vertex's own cube, per
+// axis forward unless weight + h reaches the cube's far face; h = 0.1 * cell
+// is added to the cell-unit weights as written.  No root-vertex test.
+__device__ __forceinline__ bool sdf_collide_mixed_vertex(const SdfGrid& g, const float* __restrict__ sdf,
+                                                         float margin, f3 p, f3& out) {
+    const float INITIAL = 1e10f;
+    int cx, cy, cz;
+    sdf_coords(g, p, cx, cy, cz);
+    if (!(0 <= cx && cx < g.nx - 2) || !(0 <= cy && cy < g.ny - 2) || !(0 <= cz && cz < g.nz - 2)) return false;
+    const int base = sdf_index(g, cx, cy, cz);
+    const int X = 1, Y = g.nx, Z = g.ny * g.nx;
+    const int ids[8] = {base, base + X, base + Y, base + Y + X, base + Z, base + Z + X, base + Z + Y, base + Z + Y + X};
+    float d[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] = ldg(&sdf[ids[j]]);
+    bool outside = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) outside |= (d[j] == INITIAL);
+    if (outside) return false;
+    auto tri = [&](float x, float y, float z) {
+        const float a = lerp_(lerp_(d[0], d[1], x), lerp_(d[2], d[3], x), y);
+        const float b = lerp_(lerp_(d[4], d[5], x), lerp_(d[6], d[7], x), y);
+        return lerp_(a, b, z);
+    };
+    const f3 t = divs(sub(p, sdf_cell_pos(g, cx, cy, cz)), g.cell);
+    const float dv = tri(t.x, t.y, t.z);
+    const float h = 0.1f * g.cell;
+    const f3 dir{(t.x + h < 1.0f) ? 1.0f : -1.0f, (t.y + h < 1.0f) ? 1.0f : -1.0f, (t.z + h < 1.0f) ? 1.0f : -1.0f};
+    const f3 nb{tri(t.x + h * dir.x, t.y, t.z), tri(t.x, t.y + h * dir.y, t.z), tri(t.x, t.y, t.z + h * dir.z)};
+    const f3 grad = divs(f3{dir.x * (nb.x - dv), dir.y * (nb.y - dv), dir.z * (nb.z - dv)}, h);
+    const f3 n = normalize3(grad);
+    if (dv < margin) {
+        out = add(p, mul(n, margin - dv));
+        return true;
+    }
+    return false;
+}
+
 __device__ __forceinline__ void store_xyz(float4* p, f3 v) {
     float* f = reinterpret_cast<float*>(p);
     stg(&f[0], v.x);

@@ -27,6 +27,12 @@ extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, 
                                            const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream);
 extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, int V, const float* sdf,
                                              tfx::SdfGrid g, float margin, hipStream_t stream);
+extern "C" hipError_t tfx_launch_generate_transforms(const float4* rest, int G, int F, int V,
+                                                     const tfx_hair_to_triangle_mapping* map, const float4* mesh,
+                                                     const float4* skinned, int nverts, tfx_strand_transform* out,
+                                                     hipStream_t stream);
+extern "C" hipError_t tfx_launch_sdf_collide_mixed(float4* pos, float4* prev, int n, const float* sdf, tfx::SdfGrid g,
+                                                   float margin, hipStream_t stream);
 extern "C" hipError_t tfx_launch_fused(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
                                        int nitems, int grid, hipStream_t stream);
 extern "C" int tfx_fused_block_vertices(int V, int variant);
@@ -182,6 +188,7 @@ struct tfx_hair {
     int tangent_w_uniform = 1;
     float tangent_w = 0.0f;
     Staging staging;
+    hipEvent_t handoff = nullptr;         // sim <-> rendering stream barrier (created on first use)
 };
 
 struct tfx_simulation {
@@ -211,6 +218,7 @@ struct tfx_sdf {
     uint32_t* brick_bits = nullptr;
     int bx = 0, by = 0, bz = 0;
     int device = 0;
+    int collide_variant = TFX_SDF_COLLIDE_FORWARD;
 };
 
 struct tfx_mesh_skinning {
@@ -273,6 +281,7 @@ inline void hair_free(tfx_hair* h) {
     (void)hipFree(h->pos); (void)hipFree(h->prev); (void)hipFree(h->prevprev); (void)hipFree(h->tangent);
     (void)hipFree(h->sld); (void)hipFree(h->rest); (void)hipFree(h->restlen); (void)hipFree(h->follow_off);
     (void)hipFree(h->skin); (void)hipFree(h->bones_d); (void)hipFree(h->params_d);
+    if (h->handoff) (void)hipEventDestroy(h->handoff);
 }
 
 inline void compute_tangent_w(tfx_hair* h, const float* tangents) {

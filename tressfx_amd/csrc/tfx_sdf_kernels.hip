@@ -319,6 +319,20 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_collide(float4* __restrict__ 
     }
 }
 
+// CollideHairVerticesWithSdf :597-715 — one lane per vertex, every vertex.
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_collide_mixed(float4* __restrict__ pos, float4* __restrict__ prev,
+                                                                 int n, const float* __restrict__ sdf, SdfGrid g,
+                                                                 float margin) {
+    const int i = blockIdx.x * kSdfBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 hv = pos[i];
+    f3 out;
+    if (sdf_collide_mixed_vertex(g, sdf, margin, xyz(hv), out)) {
+        store_xyz(&pos[i], out);
+        store_xyz(&prev[i], out);
+    }
+}
+
 }  // namespace tfx
 
 // phase 0 = Initialize, 1 = Construct, 2 = Finalize, 3 = pack the brick occupancy to bits
@@ -351,6 +365,15 @@ extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, 
     } else {
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t tfx_launch_sdf_collide_mixed(float4* pos, float4* prev, int n, const float* sdf, tfx::SdfGrid g,
+                                                   float margin, hipStream_t stream) {
+    using namespace tfx;
+    if (n <= 0) return hipSuccess;
+    const int blocks = (n + kSdfBlock - 1) / kSdfBlock;
+    hipLaunchKernelGGL(k_sdf_collide_mixed, dim3(blocks), dim3(kSdfBlock), 0, stream, pos, prev, n, sdf, g, margin);
     return hipGetLastError();
 }
 

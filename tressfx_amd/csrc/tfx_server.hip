@@ -354,6 +354,9 @@ int launch_pass(tfx_server* s, Pass& pass) {
             if (!r->wind.empty()) st.wind_magnitude = r->wind[at];
             rc = tfx_hair_update_simulation_parameters(h, &st, in->dt);
             if (rc) return rc;
+            if (in->sdf && in->sdf->collide_variant != TFX_SDF_COLLIDE_FORWARD)
+                return fail(TFX_ERR_UNSUPPORTED, "tfx_server: instance %d has a mixed-variant SDF; the served form "
+                                                 "collides with forward differences only", in->id);
             pack_dev_params(h, in->sdf, params[m]);
             std::memcpy(bone_block + hair_bone_off[m], h->bones_h.data(), (size_t)16 * h->bones_used * sizeof(float));
             h->bones_dirty = TFX_MAX_NUM_BONES;

@@ -242,6 +242,86 @@ __global__ void __launch_bounds__(kBlock) k_follow(DevHair h) {
     }
 }
 
+// GenerateTransforms, TressFXSimulation.hlsl:1035-1142
+// (USE_MESH_BASED_HAIR_TRANSFORM): one lane per guide strand.  Row-major 4x4
+// products summed left to right.
+struct M4 { float m[4][4]; };
+
+__device__ __forceinline__ M4 m4_mul(const M4& a, const M4& b) {
+    M4 c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            c.m[i][j] = ((a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j]) + a.m[i][3] * b.m[3][j];
+    return c;
+}
+
+__device__ __forceinline__ M4 m4_triangle_frame(f3 v1, f3 v2, f3 v3) {
+    const f3 tangent = normalize3(sub(v1, v3));
+    const f3 tangent2 = sub(v2, v3);
+    const f3 normal = normalize3(cross3(tangent, tangent2));
+    const f3 binormal = normalize3(cross3(normal, tangent));
+    M4 m;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m.m[i][j] = 0.0f;
+    m.m[0][0] = tangent.x; m.m[0][1] = tangent.y; m.m[0][2] = tangent.z;
+    m.m[1][0] = normal.x; m.m[1][1] = normal.y; m.m[1][2] = normal.z;
+    m.m[2][0] = binormal.x; m.m[2][1] = binormal.y; m.m[2][2] = binormal.z;
+    m.m[3][3] = 1.0f;
+    return m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_generate_transforms(const float4* __restrict__ rest, int G, int F, int V,
+                                                                const tfx_hair_to_triangle_mapping* __restrict__ map,
+                                                                const float4* __restrict__ mesh,
+                                                                const float4* __restrict__ skinned, int nverts,
+                                                                tfx_strand_transform* __restrict__ out) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= G) return;
+    const size_t s = (size_t)g * (F + 1);
+    const tfx_hair_to_triangle_mapping mp = map[s];
+    const long long tri = (long long)mp.triangle_index * 3;
+    if (tri + 2 >= (long long)nverts) return;
+    const float a = mp.barycentric[0], b = mp.barycentric[1], c = mp.barycentric[2];
+    const f3 v1 = xyz(mesh[tri]), v2 = xyz(mesh[tri + 1]), v3 = xyz(mesh[tri + 2]);
+    const f3 s1 = xyz(skinned[tri]), s2 = xyz(skinned[tri + 1]), s3 = xyz(skinned[tri + 2]);
+    const f3 tfmPos = add(add(mul(s1, a), mul(s2, b)), mul(s3, c));
+    const f3 initialPos = xyz(rest[s * V]);
+    const M4 triM = m4_triangle_frame(v1, v2, v3);
+    const M4 tfmTriM = m4_triangle_frame(s1, s2, s3);
+    M4 triT;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) triT.m[i][j] = triM.m[j][i];
+    M4 rot = m4_mul(triT, tfmTriM);
+    M4 tr;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tr.m[i][j] = (i == j) ? 1.0f : 0.0f;
+    tr.m[3][0] = -initialPos.x; tr.m[3][1] = -initialPos.y; tr.m[3][2] = -initialPos.z;
+    rot = m4_mul(tr, rot);
+    tr.m[3][0] = tfmPos.x; tr.m[3][1] = tfmPos.y; tr.m[3][2] = tfmPos.z;
+    const M4 tfm = m4_mul(rot, tr);
+    float q[4];
+    q[3] = sqrtf(((1.0f + tfm.m[0][0]) + tfm.m[1][1]) + tfm.m[2][2]) / 2.0f;
+    q[0] = (tfm.m[2][1] - tfm.m[1][2]) / (4.0f * q[3]);
+    q[1] = (tfm.m[0][2] - tfm.m[2][0]) / (4.0f * q[3]);
+    q[2] = (tfm.m[1][0] - tfm.m[0][1]) / (4.0f * q[3]);
+    tfx_strand_transform o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o.tfm[4 * i + j] = tfm.m[i][j];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o.quat[k] = q[k];
+    out[s] = o;
+}
+
 }  // namespace tfx
 
 // Launchers used by the C-ABI layer.
@@ -260,5 +340,16 @@ extern "C" hipError_t tfx_launch_stage(int stage, const tfx::DevHair* h, hipStre
         case 5: if (h->F > 0) hipLaunchKernelGGL(k_follow, dim3(vblocks), dim3(kBlock), 0, stream, *h); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t tfx_launch_generate_transforms(const float4* rest, int G, int F, int V,
+                                                     const tfx_hair_to_triangle_mapping* map, const float4* mesh,
+                                                     const float4* skinned, int nverts, tfx_strand_transform* out,
+                                                     hipStream_t stream) {
+    using namespace tfx;
+    if (G <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_generate_transforms, dim3((G + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, rest, G, F, V,
+                       map, mesh, skinned, nverts, out);
     return hipGetLastError();
 }
