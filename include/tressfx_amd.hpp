@@ -155,6 +155,14 @@ public:
         check(tfx_hair_set_collision_capsules(h_, centerAndRadius0, centerAndRadius1, numCapsules));
     }
     void ResetPositions() { check(tfx_hair_reset_positions(h_)); }
+    // TressFXDynamicState::TransitionSimToRendering / TransitionRenderingToSim
+    // (TressFXHairObject.cpp:559-583): later work on toStream waits for fromStream.
+    void TransitionSimToRendering(void* fromStream, void* toStream) {
+        check(tfx_hair_transition_sim_to_rendering(h_, fromStream, toStream));
+    }
+    void TransitionRenderingToSim(void* fromStream, void* toStream) {
+        check(tfx_hair_transition_rendering_to_sim(h_, fromStream, toStream));
+    }
     void IncreaseSimulationFrame() { check(tfx_hair_increase_simulation_frame(h_)); }
     tfx_dynamic_state GetDynamicState() const {
         tfx_dynamic_state s{};
@@ -317,6 +325,8 @@ public:
         check(tfx_sdf_set_mesh(s_, &d));
         check(tfx_sdf_update(s_, system.handle(), stream));
     }
+    // TFX_SDF_COLLIDE_FORWARD (the reference's dispatch) or TFX_SDF_COLLIDE_MIXED.
+    void SetCollideVariant(int variant) { check(tfx_sdf_set_collide_variant(s_, variant)); }
     void CollideWithHair(void* stream, TressFXSDFCollisionSystem& system, TressFXHairObject& hairObject) {
         check(tfx_sdf_collide_with_hair(s_, system.handle(), stream, hairObject.handle()));
     }

@@ -184,6 +184,14 @@ int main(int argc, char** argv) {
         }
         const size_t n4 = (size_t)G * (F + 1) * V * 4;
         std::vector<float> pa(n4), pb(n4), qa(n4), qb(n4);
+        // hand the strands to a renderer on another stream, and back
+        hipStream_t render = nullptr;
+        REQUIRE(hipStreamCreateWithFlags(&render, hipStreamNonBlocking) == hipSuccess);
+        hairA.TransitionSimToRendering(nullptr, render);
+        hairA.TransitionRenderingToSim(render, nullptr);
+        REQUIRE(hipStreamSynchronize(render) == hipSuccess);
+        REQUIRE(hipStreamDestroy(render) == hipSuccess);
+        sdfA.SetCollideVariant(TFX_SDF_COLLIDE_FORWARD);
         tfx_dynamic_state st = hairA.GetDynamicState();
         REQUIRE(st.positions != nullptr);
         check(tfx_hair_download_state(hairA.handle(), nullptr, pa.data(), qa.data(), nullptr, nullptr, nullptr));
