@@ -247,6 +247,80 @@ def test_collide_and_fused_vs_oracle(gpu):
     assert hits > 0, "the SDF never moved a vertex: the test does not exercise the collision"
 
 
+def test_collide_gradient_lookup_cases_vs_oracle(gpu):
+    """CollideWithHair on placed vertices, bit-exact against the FP32 oracle.
+    The forward-difference lookups at p + offset * h share the centre's cell
+    on the unmoved axes and step at most one cell on the moved one; the
+    vertices here hit every branch of that: steps inside the cell and into the
+    next, steps out of the grid's lookup range (the far face of the last
+    usable cell), coordinates exactly on cell boundaries or one step below
+    them, -0 coordinates (where p + 0 * h changes the bits, taking the lookups
+    as written) and plain near-surface vertices."""
+    torch = gpu
+    rng = np.random.default_rng(3)
+    mesh = synth.NoisySphereMesh(rng, radius=0.1, subdivisions=4, amp=0.0, anim_hz=0.0)
+    dm = DeviceMesh(torch, mesh)
+    desc = dm.update(0.0)
+    n, cell = 44, np.float32(0.005)
+    origin = np.full(3, -0.11, np.float32)
+    sdf = tfx.TressFXSDFCollision(desc, collision_margin=1.0, explicit=(tuple(origin), float(cell), (n, n, n)))
+    system = tfx.TressFXSDFCollisionSystem()
+    sdf.Update(system)
+    grid32 = sdf.download()
+    h = np.float32(0.1) * cell
+    G, V = 320, 16
+    roots, nrm = synth.hemisphere_roots(G, 0.1, rng)
+    asset = synth.build_asset(synth.grow_strands(roots, nrm, V, 0.005, 0.15, rng), 0, rng)
+    hair = tfx.TressFXHairObject(asset)
+    st = hair.download_state()
+    pts = []
+    # the last usable cell is n - 3; its far face is where the +h step leaves the range
+    edge = origin[0] + np.float32(n - 2) * cell
+    for k in range(3):
+        for x in np.linspace(edge - np.float32(2.0) * h, edge - np.float32(1e-6), 24, dtype=np.float32):
+            q = np.zeros(3, np.float32)
+            q[k] = x
+            pts.append(q)
+    for k in range(3):                                        # cell boundaries and one step below them
+        for j in range(38, 42):
+            b = origin[k] + np.float32(j) * cell
+            for x in (b, b - h, np.nextafter(b, np.float32(0.0)), np.nextafter(b - h, np.float32(1.0))):
+                q = np.zeros(3, np.float32)
+                q[k] = x
+                q[(k + 1) % 3] = np.float32(0.002)
+                pts.append(q)
+    for k in range(3):                                        # -0 on the other axes
+        for r in np.linspace(0.096, 0.1005, 16, dtype=np.float32):
+            for sgn in (1.0, -1.0):
+                q = np.full(3, -0.0, np.float32)
+                q[k] = np.float32(sgn) * r
+                pts.append(q)
+    d = rng.normal(size=(4000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pts += list((d * rng.uniform(0.096, 0.104, (4000, 1))).astype(np.float32))
+    pts = np.asarray(pts, np.float32)
+    usable = np.arange(G * V).reshape(G, V)[:, 2:].reshape(-1)
+    assert len(pts) <= len(usable)
+    pos = st["pos"].copy()
+    pos[usable[:len(pts)], :3] = pts
+    prev = pos.copy()
+    prev[:, :3] += np.float32(1e-4)
+    st["pos"], st["prev"] = pos, prev
+    hair.upload_state(st)
+    sdf.CollideWithHair(system, hair)
+    got = hair.download_state()
+    cp = sdf.collision_params(hair)
+    rp, rq = pos.copy(), prev.copy()
+    oracle.sdf_collide(rp, rq, V, grid32, origin, cell, (n, n, n), np.float32(cp.collision_margin))
+    for k, ref in (("pos", rp), ("prev", rq)):
+        mm = first_mismatch(got[k], ref)
+        assert mm is None, f"{k} differs at {mm[0]} gpu={mm[1]!r} ref={mm[2]!r} ({mm[3]} words)"
+    moved = (rp != pos).any(axis=1)
+    assert moved.sum() > 2000
+    negz = np.signbit(pts) & (pts == 0)
+    assert moved[usable[:len(pts)]][negz.any(axis=1)].sum() > 20, "no -0 vertex reached the gradient lookups"
+
+
 def test_collide_mixed_difference_variant_vs_oracle(gpu):
     """CollideWithHair with the mixed forward/backward-difference kernel
     (TressFXSDFCollision.hlsl:597-715, every vertex including the roots),

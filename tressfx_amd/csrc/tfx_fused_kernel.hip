@@ -472,22 +472,32 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 __syncthreads();
             }
             const int nq = staged ? sQueueCount : nf;
+            // Store index and tangent of follow vertex idx, recomputed after the
+            // projection rather than held across it.
+            auto follow_slot = [&](int idx, size_t& fi, float4& ft) {
+                const int j = idx / per + 1;
+                const int lv = idx - (j - 1) * per;
+                const int ls = lv / V, v = lv % V;
+                fi = ((size_t)(g0 + ls) * (F + 1) + j) * V + v;
+                ft = sVB[ls * STRIDE + v];
+            };
             for (int q0 = t; q0 < nq; q0 += 4 * kFusedBlock) {
                 f3 fp[4], outp[4];
                 bool act[4], hit[4], ok[4];
-                size_t fi[4];
-                float4 ft[4];
+                int qi[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int q = q0 + u * kFusedBlock;
                     ok[u] = q < nq;
                     act[u] = false;
                     fp[u] = f3{0.f, 0.f, 0.f};
-                    fi[u] = 0;
-                    ft[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    qi[u] = 0;
                     if (ok[u]) {
                         int v;
-                        follow_vertex(staged ? (int)queue[q] : q, fi[u], fp[u], ft[u], v);
+                        size_t fi;
+                        float4 ft;
+                        qi[u] = staged ? (int)queue[q] : q;
+                        follow_vertex(qi[u], fi, fp[u], ft, v);
                         act[u] = has_sdf && v >= 2 && (staged || maybe_near(fp[u]));
                     }
                 }
@@ -495,9 +505,12 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     if (!ok[u]) continue;
-                    if (hit[u]) store_xyz(&h.prev[fi[u]], outp[u]);
-                    store_xyz(&h.pos[fi[u]], hit[u] ? outp[u] : fp[u]);
-                    stg(&h.tangent[fi[u]], ft[u]);
+                    size_t fi;
+                    float4 ft;
+                    follow_slot(qi[u], fi, ft);
+                    if (hit[u]) store_xyz(&h.prev[fi], outp[u]);
+                    store_xyz(&h.pos[fi], hit[u] ? outp[u] : fp[u]);
+                    stg(&h.tangent[fi], ft);
                 }
             }
         }
