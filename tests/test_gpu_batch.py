@@ -126,6 +126,73 @@ def test_batch_graph_recaptures_on_substep_change(gpu):
          stream_of=lambda c: side.cuda_stream, graph_of=lambda c: True)
 
 
+def test_batch_sparse_sdf_build_and_shared_sdf(gpu):
+    """The batched SDF build initialises and finalises only the bricks that the
+    triangles' node boxes overlap (resetting the previous build's others); its
+    grid must equal the per-object build of the same mesh after every frame,
+    with the mesh rotating and deforming, the grid origin following its
+    bounding box, graph and direct frames alternating, and a per-object build
+    of the batched SDF in between (which drops the band history).  Two hair
+    objects share that SDF in the batch (one build per frame) and must equal
+    the per-object dispatch colliding both with one SDF."""
+    torch = gpu
+    print("[batch] sparse SDF build, shared SDF, 8 frames", flush=True)
+    specs = [(120, 16, 0), (90, 32, 1)]
+    wls = [small_workload(G, V, F, seed=400 + i) for i, (G, V, F) in enumerate(specs)]
+    rng = np.random.Generator(np.random.PCG64(77))
+    mesh = synth.NoisySphereMesh(rng, radius=0.1, subdivisions=3, amp=0.015, anim_hz=2.0)
+    dm = DeviceMesh(torch, mesh, synth.head_yaw)
+    desc = dm.update(0.0)
+    sdf_ref, sdf_b = (tfx.TressFXSDFCollision(desc, num_cells_in_x=40, collision_margin=1.0) for _ in range(2))
+    hairs_ref = [make_hair(w) for w in wls]
+    hairs_b = [make_hair(w) for w in wls]
+    batch = tfx.HairBatch()
+    for h in hairs_b:
+        batch.add(h, sdf_b)
+    system = tfx.TressFXSDFCollisionSystem()
+    sim = tfx.TressFXSimulation()
+    side = torch.cuda.Stream()
+    members = [Member.__new__(Member) for _ in wls]
+    for m, w in zip(members, wls):
+        m.wl = w
+    prev, changed = None, 0
+    for frame in range(8):
+        desc = dm.update(0.1 * frame)        # 0.1 s apart: the head turns and the band moves
+        sdf_ref.SetMesh(desc)
+        sdf_b.SetMesh(desc)
+        torch.cuda.synchronize()
+        if frame == 5:
+            sdf_b.Update(system)
+        sdf_ref.Update(system)
+        for sub in range(2):
+            for h, w in zip(hairs_ref, wls):
+                w.substeps = 2
+                set_substep_inputs(h, w, frame, sub)
+            sim.Simulate(hairs_ref)
+            for h in hairs_ref:
+                sdf_ref.CollideWithHair(system, h)
+        batch.begin_frame()
+        for sub in range(2):
+            ins = [m.substep_inputs(frame, sub) for m in members]
+            bones = np.stack([x[0] for x in ins])
+            settings = (tfx.SimulationSettings * len(members))(*[x[1] for x in ins])
+            batch.stage_substep(bones, settings, np.float32(1.0 / synth.FRAME_RATE / 2))
+        batch.run_frame(side.cuda_stream, use_graph=frame % 2 == 1)
+        torch.cuda.synchronize()
+        ref = sdf_ref.download()
+        mm = first_mismatch(sdf_b.download(), ref)
+        assert mm is None, f"frame {frame}: batched SDF differs at cell {mm[0]}: {mm[1]!r} vs {mm[2]!r} ({mm[3]} cells)"
+        if prev is not None:
+            changed += int((prev != ref).sum())
+        prev = ref
+        for i, (h, hr) in enumerate(zip(hairs_b, hairs_ref)):
+            got, want = h.download_state(), hr.download_state()
+            for k in ("pos", "prev", "tangent"):
+                mm = first_mismatch(got[k], want[k])
+                assert mm is None, f"frame {frame} hair {i} {k}: {mm}"
+    assert changed > 10000, "the mesh barely moved: the band history is not exercised"
+
+
 def test_batch_membership_change(gpu):
     """Removing and adding members re-lays out the batch; the remaining
     members keep stepping exactly."""

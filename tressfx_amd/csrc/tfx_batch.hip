@@ -35,7 +35,7 @@ struct tfx_batch {
     size_t off_sub = 0, sub_params = 0, sub_bytes = 0, slot_bytes = 0;
     std::vector<int> bone_off;         // per member, floats into one substep's bone block
     int bone_floats = 0;
-    std::vector<int> sdf_members;      // member indices that own an SDF
+    std::vector<int> sdf_members;      // per distinct SDF, the first member that collides with it
     int max_cells = 0, max_tris = 0;
 
     void* dev_slot = nullptr;          // device copy of a slot
@@ -92,7 +92,12 @@ static int prepare(tfx_batch* b) {
     for (int m = 0; m < n; ++m) {
         b->bone_off[m] = b->bone_floats;
         b->bone_floats += 16 * b->members[m].hair->bones_used;
-        if (tfx_sdf* s = b->members[m].sdf) {
+        // One build per distinct SDF: members may share one (several hair
+        // objects colliding with one head), and it is built once per frame.
+        tfx_sdf* s = b->members[m].sdf;
+        bool first = s != nullptr;
+        for (int j : b->sdf_members) first = first && b->members[j].sdf != s;
+        if (first) {
             b->sdf_members.push_back(m);
             b->max_cells = std::max(b->max_cells, s->nx * s->ny * s->nz);
         }
@@ -326,32 +331,42 @@ extern "C" int tfx_batch_run_frame(tfx_batch* b, void* stream, int use_graph) {
         b->slot = -1;
         return TFX_OK;
     }
+    // Band pointers of each SDF build (the sparse initialise/finalise), in
+    // launch order; a frame that fails to launch leaves the history unknown.
+    tfx::SdfJob* jobs = reinterpret_cast<tfx::SdfJob*>(b->host[k]);
+    for (size_t j = 0; j < b->sdf_members.size(); ++j) sdf_band_stage(b->members[b->sdf_members[j]].sdf, jobs[j]);
+    auto failed = [&](int code) {
+        for (int j : b->sdf_members) b->members[j].sdf->band_valid = 0;
+        b->slot = -1;
+        return code;
+    };
     if (use_graph) {
         CHECK_ARG(s != nullptr, "tfx_batch_run_frame: graph replay needs a non-default stream");
         if (!b->exec[k] || b->exec_substeps[k] != substeps || b->exec_tris[k] != b->max_tris) {
             if (b->exec[k]) { (void)hipGraphExecDestroy(b->exec[k]); b->exec[k] = nullptr; }
-            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return failed(fail(TFX_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e)));
             rc = launch_frame(b, s, substeps, 0);
             hipGraph_t graph = nullptr;
-            hipError_t e = hipStreamEndCapture(s, &graph);
-            if (rc) { if (graph) (void)hipGraphDestroy(graph); b->slot = -1; return rc; }
-            if (e != hipSuccess) { b->slot = -1; return fail(TFX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e)); }
+            e = hipStreamEndCapture(s, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return failed(rc); }
+            if (e != hipSuccess) return failed(fail(TFX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e)));
             e = hipGraphInstantiate(&b->exec[k], graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) { b->slot = -1; return fail(TFX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+            if (e != hipSuccess) return failed(fail(TFX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)));
             b->exec_substeps[k] = substeps;
             b->exec_tris[k] = b->max_tris;
         }
         PassTimer tm("BatchFrame", s);
         hipError_t e = hipGraphLaunch(b->exec[k], s);
-        if (e != hipSuccess) { b->slot = -1; return fail(TFX_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e)); }
+        if (e != hipSuccess) return failed(fail(TFX_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e)));
     } else {
         PassTimer tm("BatchFrame", s);
         rc = launch_frame(b, s, substeps, 1);
     }
+    if (rc) return failed(rc);
     hipError_t e = hipEventRecord(b->ev[k], s);
     b->slot = -1;
-    if (rc) return rc;
     if (e != hipSuccess) return fail(TFX_ERR_HIP, "hipEventRecord: %s", hipGetErrorString(e));
     return TFX_OK;
 }

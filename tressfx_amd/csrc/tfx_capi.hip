@@ -544,6 +544,10 @@ static int sdf_alloc(tfx_sdf* s) {
     e = hipMalloc((void**)&s->brick_bits, ((nbrick + 31) / 32) * sizeof(uint32_t));
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_sdf: %s", hipGetErrorString(e));
+    e = hipMalloc((void**)&s->band, 2 * nbrick);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_sdf: %s", hipGetErrorString(e));
+    HIP_TRY(hipMemset(s->band, 0, 2 * nbrick));
     // Until the first Update every brick is "maybe occupied": lookups stay exact.
     HIP_TRY(hipMemset(s->bricks, 1, nbrick));
     HIP_TRY(hipMemset(s->brick_bits, 0xff, ((nbrick + 31) / 32) * sizeof(uint32_t)));
@@ -606,6 +610,7 @@ extern "C" int tfx_sdf_destroy(tfx_sdf* s) {
     (void)hipFree(s->cells);
     (void)hipFree(s->bricks);
     (void)hipFree(s->brick_bits);
+    (void)hipFree(s->band);
     delete s;
     return TFX_OK;
 }
@@ -623,6 +628,7 @@ extern "C" int tfx_sdf_update(tfx_sdf* s, tfx_sdf_collision_system* sys, void* s
     if (!s->explicit_grid)
         for (int k = 0; k < 3; ++k) s->origin[k] = s->mesh.bbox_min[k] - s->pad_boundary;  // UpdateSDFGrid
     const int ncell = s->nx * s->ny * s->nz;
+    s->band_valid = 0;  // the whole grid is rewritten: a batched build's band history no longer applies
     static const char* kNames[4] = {"InitializeSignedDistanceField", "ConstructSignedDistanceField",
                                     "FinalizeSignedDistanceField", "FinalizeSignedDistanceField"};
     for (int phase = 0; phase < 4; ++phase) {

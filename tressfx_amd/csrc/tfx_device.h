@@ -376,6 +376,13 @@ struct SdfJob {
     int ntri;
     int ncell;
     SdfGrid g;
+    // Sparse bookkeeping, per 4^3-cell brick (the occupancy map's bricks):
+    // band_cur receives the bricks this build's construct pass can touch;
+    // band_prev holds those of the grid's previous build (null: unknown, every
+    // other brick is reset).  Null band_cur: the whole grid is initialised
+    // and finalised.
+    uint8_t* band_cur;
+    const uint8_t* band_prev;
 };
 
 // One mesh of a batched skinning pass (TressFXBoneSkinning.hlsl:57-94).

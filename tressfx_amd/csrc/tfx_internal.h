@@ -219,7 +219,28 @@ struct tfx_sdf {
     int bx = 0, by = 0, bz = 0;
     int device = 0;
     int collide_variant = TFX_SDF_COLLIDE_FORWARD;
+    // Bricks touched by the batched builds (tfx::SdfJob::band_cur), two per
+    // brick: the build in flight writes band[band_parity], the previous one's
+    // is the other.  band_valid = 0 once anything else has written the grid.
+    uint8_t* band = nullptr;
+    int band_parity = 0;
+    int band_valid = 0;
 };
+
+// The band pointers of the next batched build of s, advancing its history.
+// Called once per build, in build order, right before it is launched.
+inline void sdf_band_stage(tfx_sdf* s, tfx::SdfJob& jb) {
+    if (!s->band) {
+        jb.band_cur = nullptr;
+        jb.band_prev = nullptr;
+        return;
+    }
+    const size_t nbrick = (size_t)s->bx * s->by * s->bz;
+    jb.band_cur = s->band + (s->band_parity ? nbrick : 0);
+    jb.band_prev = s->band_valid ? s->band + (s->band_parity ? 0 : nbrick) : nullptr;
+    s->band_parity ^= 1;
+    s->band_valid = 1;
+}
 
 struct tfx_mesh_skinning {
     int n = 0, ntri = 0, follow_bone = 0;

@@ -236,23 +236,18 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_bricks(const uint8_t* __
     bits[w] = pack_brick_word(bricks, nbrick, w);
 }
 
-// ConstructSignedDistanceField :329-397 for one triangle, spread over the
-// kTeam lanes of a team: the triangle's padded node box (AABB +- one cell,
-// GetSdfCoordinates, -1/+1, clamped) walked in x-fastest order.  A node's
-// current value is read first (agent-coherent load) and the atomicMin issued
-// only if the new value is smaller: the field only decreases, so a value read
-// at any time bounds the final one from above and skipping is exact.
-template <int kTeam>
-__device__ __forceinline__ void construct_triangle(uint32_t* __restrict__ cells, const tfx_standard_vertex* verts,
-                                                   const uint32_t* idx, int tri, const SdfGrid& g, int lane) {
+// Triangle `tri`'s vertices and its padded node box (ConstructSignedDistanceField
+// :329-397: AABB +- one cell, GetSdfCoordinates, -1/+1, clamped to the grid).
+__device__ __forceinline__ void triangle_node_box(const tfx_standard_vertex* verts, const uint32_t* idx, int tri,
+                                                  const SdfGrid& g, f3& x0, f3& x1, f3& x2, int (&lo)[3],
+                                                  int (&hi)[3]) {
     const uint32_t i0 = idx[3 * tri + 0], i1 = idx[3 * tri + 1], i2 = idx[3 * tri + 2];
-    const f3 x0{verts[i0].position[0], verts[i0].position[1], verts[i0].position[2]};
-    const f3 x1{verts[i1].position[0], verts[i1].position[1], verts[i1].position[2]};
-    const f3 x2{verts[i2].position[0], verts[i2].position[1], verts[i2].position[2]};
+    x0 = f3{verts[i0].position[0], verts[i0].position[1], verts[i0].position[2]};
+    x1 = f3{verts[i1].position[0], verts[i1].position[1], verts[i1].position[2]};
+    x2 = f3{verts[i2].position[0], verts[i2].position[1], verts[i2].position[2]};
     const float m = g.cell;  // MARGIN
     f3 mn{fmn(x0.x, fmn(x1.x, x2.x)) - m, fmn(x0.y, fmn(x1.y, x2.y)) - m, fmn(x0.z, fmn(x1.z, x2.z)) - m};
     f3 mx{fmx(x0.x, fmx(x1.x, x2.x)) + m, fmx(x0.y, fmx(x1.y, x2.y)) + m, fmx(x0.z, fmx(x1.z, x2.z)) + m};
-    int lo[3], hi[3];
     sdf_coords(g, mn, lo[0], lo[1], lo[2]);
     sdf_coords(g, mx, hi[0], hi[1], hi[2]);
     const int dims[3] = {g.nx, g.ny, g.nz};
@@ -263,8 +258,25 @@ __device__ __forceinline__ void construct_triangle(uint32_t* __restrict__ cells,
         lo[k] = max(0, min(lo[k], dims[k] - 1));
         hi[k] = max(0, min(hi[k], dims[k] - 1));
     }
+}
+
+// ConstructSignedDistanceField :329-397 for one triangle, spread over the
+// kTeam lanes of a team: the triangle's padded node box walked in x-fastest
+// order.  A node's current value is read first (agent-coherent load) and the
+// atomicMin issued only if the new value is smaller: the field only
+// decreases, so a value read at any time bounds the final one from above and
+// skipping is exact.
+template <int kTeam>
+__device__ __forceinline__ void construct_triangle(uint32_t* __restrict__ cells, const tfx_standard_vertex* verts,
+                                                   const uint32_t* idx, int tri, const SdfGrid& g, int lane) {
+    f3 x0, x1, x2;
+    int lo[3], hi[3];
+    triangle_node_box(verts, idx, tri, g, x0, x1, x2, lo, hi);
     const TriPrep tp = tri_prep(x0, x1, x2);
-    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    // An axis whose wrapped lower bound passes the upper one (a vertex more
+    // than 2^31 cells below the origin) has no nodes, as in the reference's
+    // nested loops.
+    const int ex = max(hi[0] - lo[0] + 1, 0), ey = max(hi[1] - lo[1] + 1, 0), ez = max(hi[2] - lo[2] + 1, 0);
     const int total = ex * ey * ez;
     // Exact small-integer division through the float reciprocal: for
     // 0 <= c < 2^16, (c + 0.5) / e stays more than 0.5/e - 2^-8/e from any
@@ -393,12 +405,90 @@ extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, i
 // ---------------------------------------------------------------------------
 namespace tfx {
 
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_init_multi(const SdfJob* __restrict__ jobs) {
+// Brick of node i of a whole-grid pass.
+__device__ __forceinline__ int node_brick(int i, const SdfGrid& g, float inv_nx, float inv_ny, bool fast) {
+    int x, y, z, r;
+    if (fast) {
+        divmod24(i, g.nx, inv_nx, r, x);
+        divmod24(r, g.ny, inv_ny, z, y);
+    } else {
+        x = i % g.nx;
+        r = i / g.nx;
+        y = r % g.ny;
+        z = r / g.ny;
+    }
+    return ((z >> kBrickShift) * g.by + (y >> kBrickShift)) * g.bx + (x >> kBrickShift);
+}
+
+// The batched build's sparse form (SdfJob::band_cur).  Only bricks that a
+// triangle's node box overlaps can leave Construct with a value other than
+// FloatFlip3(1e10), so after Finalize every other node holds 1e10 — the value
+// it already holds if it was outside the previous build's band as well.
+// Initialize therefore writes FloatFlip3(1e10) into this build's band, resets
+// the previous band's other bricks to 1e10 and leaves the rest; Finalize
+// un-flips this build's band only.  The grid equals the full build's.
+
+// Step 1: clear this build's band and the occupancy map.
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_band_clear_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
-    init_cells(j.cells, j.ncell, blockIdx.x * kSdfBlock + threadIdx.x, gridDim.x * kSdfBlock);
     const int nbrick = j.g.bx * j.g.by * j.g.bz;
     uint8_t* bricks = const_cast<uint8_t*>(j.g.bricks);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < nbrick; i += gridDim.x * kSdfBlock) bricks[i] = 0;
+    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < nbrick; i += gridDim.x * kSdfBlock) {
+        if (bricks) bricks[i] = 0;
+        if (j.band_cur) j.band_cur[i] = 0;
+    }
+}
+
+// Step 2: every brick a triangle's node box overlaps (the box Construct walks).
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_band_mark_multi(const SdfJob* __restrict__ jobs) {
+    const SdfJob& j = jobs[blockIdx.y];
+    if (!j.band_cur) return;
+    const SdfGrid g = j.g;
+    for (int tri = blockIdx.x * kSdfBlock + threadIdx.x; tri < j.ntri; tri += gridDim.x * kSdfBlock) {
+        f3 x0, x1, x2;
+        int lo[3], hi[3];
+        triangle_node_box(j.verts, j.idx, tri, g, x0, x1, x2, lo, hi);
+        for (int bz = lo[2] >> kBrickShift; bz <= hi[2] >> kBrickShift; ++bz)
+            for (int by = lo[1] >> kBrickShift; by <= hi[1] >> kBrickShift; ++by)
+                for (int bx = lo[0] >> kBrickShift; bx <= hi[0] >> kBrickShift; ++bx)
+                    j.band_cur[(bz * g.by + by) * g.bx + bx] = 1;
+    }
+}
+
+// Step 3, InitializeSignedDistanceField :301-312 over the band (or the grid).
+// With quads (nx % 4 == 0) a lane takes one brick: its 16 quads (4 nodes along
+// x, one per y and z of the brick), so neighbouring lanes write neighbouring
+// quads and clean bricks cost one byte read.
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_init_multi(const SdfJob* __restrict__ jobs) {
+    const SdfJob& j = jobs[blockIdx.y];
+    const int lane = blockIdx.x * kSdfBlock + threadIdx.x, stride = gridDim.x * kSdfBlock;
+    if (!j.band_cur) {
+        init_cells(j.cells, j.ncell, lane, stride);
+        return;
+    }
+    const SdfGrid& g = j.g;
+    const uint32_t key = float_flip3(1e10f), flat = __float_as_uint(1e10f);
+    if ((g.nx & 3) == 0) {
+        const int nbrick = g.bx * g.by * g.bz, nq = g.nx >> 2;
+        uint4* p = reinterpret_cast<uint4*>(j.cells);
+        for (int b = lane; b < nbrick; b += stride) {
+            const bool cur = j.band_cur[b] != 0;
+            if (!cur && j.band_prev && !j.band_prev[b]) continue;
+            const uint32_t v = cur ? key : flat;
+            const int bx = b % g.bx, r = b / g.bx, by = r % g.by, bz = r / g.by;
+            for (int z = bz << kBrickShift; z < min((bz + 1) << kBrickShift, g.nz); ++z)
+                for (int y = by << kBrickShift; y < min((by + 1) << kBrickShift, g.ny); ++y)
+                    stg(&p[(z * g.ny + y) * nq + bx], make_uint4(v, v, v, v));
+        }
+    } else {
+        const bool fast = j.ncell <= (1 << 24);
+        const float inv_nx = 1.0f / (float)g.nx, inv_ny = 1.0f / (float)g.ny;
+        for (int i = lane; i < j.ncell; i += stride) {
+            const int b = node_brick(i, g, inv_nx, inv_ny, fast);
+            if (j.band_cur[b]) stg(&j.cells[i], key);
+            else if (!j.band_prev || j.band_prev[b]) stg(&j.cells[i], flat);
+        }
+    }
 }
 
 template <int kTeam>
@@ -410,9 +500,78 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob*
         construct_triangle<kTeam>(j.cells, j.verts, j.idx, tri, g, lane);
 }
 
+// Finalize of the quads of brick (bx, by, bz), nx % 4 == 0: the same
+// un-flip and the same occupancy marks as finalize_quad on each quad.  A
+// touched node marks the bricks its kBrickHalo-dilated extent meets, all within
+// one brick of its own, so the marks are gathered as a 3x3x3 mask (bit
+// 9 (dz+1) + 3 (dy+1) + (dx+1)) and each marked brick is stored once.
+__device__ __forceinline__ void finalize_brick(uint32_t* __restrict__ cells, const SdfGrid& g, int bx, int by, int bz) {
+    const int nq = g.nx >> 2;
+    uint4* p = reinterpret_cast<uint4*>(cells);
+    const uint32_t far = float_flip3(1e10f);
+    uint32_t marks = 0;
+    auto quad = [&](int y, int z, const uint4& u) {
+        const int q = (z * g.ny + y) * nq + bx;
+        stg(&p[q], make_uint4(ifloat_flip3(u.x), ifloat_flip3(u.y), ifloat_flip3(u.z), ifloat_flip3(u.w)));
+        const int m = (u.x != far ? 1 : 0) | (u.y != far ? 2 : 0) | (u.z != far ? 4 : 0) | (u.w != far ? 8 : 0);
+        if (m == 0) return;
+        const int xlo = 4 * bx + __builtin_ctz(m), xhi = 4 * bx + 31 - __builtin_clz(m);
+        const int x0 = max(xlo - kBrickHalo, 0) >> kBrickShift, x1 = min((xhi + kBrickHalo) >> kBrickShift, g.bx - 1);
+        const int y0 = max(y - kBrickHalo, 0) >> kBrickShift, y1 = min((y + kBrickHalo) >> kBrickShift, g.by - 1);
+        const int z0 = max(z - kBrickHalo, 0) >> kBrickShift, z1 = min((z + kBrickHalo) >> kBrickShift, g.bz - 1);
+        // bits lo..hi of a 3-bit row, relative to the own brick at bit 1
+        auto row = [](int lo, int hi, int own) { return ((2u << (hi - own + 1)) - 1u) & ~((1u << (lo - own + 1)) - 1u); };
+        const uint32_t xm = row(x0, x1, bx), ym = row(y0, y1, by), zm = row(z0, z1, bz);
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+                if (((zm >> dz) & 1u) && ((ym >> dy) & 1u)) marks |= xm << (9 * dz + 3 * dy);
+    };
+    const int y0 = by << kBrickShift, z0 = bz << kBrickShift;
+    if (y0 + 4 <= g.ny && z0 + 4 <= g.nz) {
+        // two rounds of eight loads in flight
+        for (int h = 0; h < 2; ++h) {
+            uint4 u[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) u[k] = ldg(&p[((z0 + 2 * h + (k >> 2)) * g.ny + y0 + (k & 3)) * nq + bx]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) quad(y0 + (k & 3), z0 + 2 * h + (k >> 2), u[k]);
+        }
+    } else {
+        for (int z = z0; z < min(z0 + 4, g.nz); ++z)
+            for (int y = y0; y < min(y0 + 4, g.ny); ++y) quad(y, z, ldg(&p[(z * g.ny + y) * nq + bx]));
+    }
+    if (!marks || !g.bricks) return;
+    uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
+    for (int k = 0; k < 27; ++k)
+        if ((marks >> k) & 1u)
+            bricks[((bz + k / 9 - 1) * g.by + (by + (k / 3) % 3 - 1)) * g.bx + (bx + k % 3 - 1)] = 1;
+}
+
+// FinalizeSignedDistanceField :400-412 over the band (or the grid), a brick
+// per lane as in the initialisation.
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
-    finalize_cells(j.cells, j.ncell, j.g, blockIdx.x * kSdfBlock + threadIdx.x, gridDim.x * kSdfBlock);
+    const int lane = blockIdx.x * kSdfBlock + threadIdx.x, stride = gridDim.x * kSdfBlock;
+    if (!j.band_cur) {
+        finalize_cells(j.cells, j.ncell, j.g, lane, stride);
+        return;
+    }
+    const SdfGrid& g = j.g;
+    if ((g.nx & 3) == 0) {
+        const int nbrick = g.bx * g.by * g.bz;
+        for (int b = lane; b < nbrick; b += stride) {
+            if (!j.band_cur[b]) continue;
+            const int bx = b % g.bx, r = b / g.bx;
+            finalize_brick(j.cells, g, bx, r % g.by, r / g.by);
+        }
+    } else {
+        const bool fast = j.ncell <= (1 << 24);
+        const float inv_nx = 1.0f / (float)g.nx, inv_ny = 1.0f / (float)g.ny;
+        for (int i = lane; i < j.ncell; i += stride)
+            if (j.band_cur[node_brick(i, g, inv_nx, inv_ny, fast)]) finalize_cell(j.cells, i, g, inv_nx, inv_ny, fast);
+    }
 }
 
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_multi(const SdfJob* __restrict__ jobs) {
@@ -432,15 +591,25 @@ extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, i
     using namespace tfx;
     if (njobs <= 0) return hipSuccess;
     const int cx = std::max(1, std::min((max_cells / 4 + kSdfBlock - 1) / kSdfBlock, 1024));
+    // a brick per lane: band bookkeeping, and init / finalize over a band (a job
+    // without band bookkeeping covers its whole grid in the same launch through
+    // the grid-stride loops)
+    const int bxg = std::max(1, std::min((max_cells / 64 + kSdfBlock - 1) / kSdfBlock, 1024));
     switch (phase) {
-        case 0: hipLaunchKernelGGL(k_sdf_init_multi, dim3(cx, njobs), dim3(kSdfBlock), 0, stream, jobs); break;
+        case 0: {
+            const int txg = std::max(1, std::min((max_tris + kSdfBlock - 1) / kSdfBlock, 1024));
+            hipLaunchKernelGGL(k_sdf_band_clear_multi, dim3(bxg, njobs), dim3(kSdfBlock), 0, stream, jobs);
+            hipLaunchKernelGGL(k_sdf_band_mark_multi, dim3(txg, njobs), dim3(kSdfBlock), 0, stream, jobs);
+            hipLaunchKernelGGL(k_sdf_init_multi, dim3(bxg, njobs), dim3(kSdfBlock), 0, stream, jobs);
+            break;
+        }
         case 1: {
             constexpr int kTeam = 16;
             const int tx = std::max(1, std::min((max_tris * kTeam + kSdfBlock - 1) / kSdfBlock, 4096));
             hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kSdfBlock), 0, stream, jobs);
             break;
         }
-        case 2: hipLaunchKernelGGL(k_sdf_finalize_multi, dim3(cx, njobs), dim3(kSdfBlock), 0, stream, jobs); break;
+        case 2: hipLaunchKernelGGL(k_sdf_finalize_multi, dim3(bxg, njobs), dim3(kSdfBlock), 0, stream, jobs); break;
         case 3: hipLaunchKernelGGL(k_sdf_pack_multi, dim3(std::max(1, cx / 32), njobs), dim3(kSdfBlock), 0, stream, jobs);
             break;
         default: return hipErrorInvalidValue;

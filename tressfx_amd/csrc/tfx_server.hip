@@ -330,6 +330,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
         jb.ntri = (sd->mesh.vertices && sd->mesh.indices) ? sd->mesh.num_triangles : 0;
         jb.ncell = sd->nx * sd->ny * sd->nz;
         jb.g = sdf_grid(sd);
+        sdf_band_stage(sd, jb);
         max_cells = std::max(max_cells, jb.ncell);
         max_tris = std::max(max_tris, jb.ntri);
     }
@@ -464,6 +465,9 @@ void worker_main(tfx_server* s) {
         const int rc = launch_pass(s, pass);
         lk.lock();
         if (rc) {
+            // the pass may have advanced SDF band histories it never built
+            for (auto& fr : pass.frames)
+                if (fr.second->sdf) fr.second->sdf->band_valid = 0;
             fail_all(s, rc, tfx_last_error());
             continue;
         }
