@@ -134,21 +134,25 @@ def test_batch_sparse_sdf_build_and_shared_sdf(gpu):
     bounding box, graph and direct frames alternating, and a per-object build
     of the batched SDF in between (which drops the band history).  Two hair
     objects share that SDF in the batch (one build per frame) and must equal
-    the per-object dispatch colliding both with one SDF."""
+    the per-object dispatch colliding both with one SDF; a third has its own
+    128^3 grid (with it the build takes four lanes per brick)."""
     torch = gpu
     print("[batch] sparse SDF build, shared SDF, 8 frames", flush=True)
-    specs = [(120, 16, 0), (90, 32, 1)]
+    specs = [(120, 16, 0), (90, 32, 1), (64, 16, 0)]
     wls = [small_workload(G, V, F, seed=400 + i) for i, (G, V, F) in enumerate(specs)]
     rng = np.random.Generator(np.random.PCG64(77))
     mesh = synth.NoisySphereMesh(rng, radius=0.1, subdivisions=3, amp=0.015, anim_hz=2.0)
     dm = DeviceMesh(torch, mesh, synth.head_yaw)
     desc = dm.update(0.0)
     sdf_ref, sdf_b = (tfx.TressFXSDFCollision(desc, num_cells_in_x=40, collision_margin=1.0) for _ in range(2))
+    big = ((-0.16, -0.16, -0.16), 0.0025, (128, 128, 128))
+    big_ref, big_b = (tfx.TressFXSDFCollision(desc, collision_margin=1.0, explicit=big) for _ in range(2))
+    colliders_ref, colliders_b = [sdf_ref, sdf_ref, big_ref], [sdf_b, sdf_b, big_b]
     hairs_ref = [make_hair(w) for w in wls]
     hairs_b = [make_hair(w) for w in wls]
     batch = tfx.HairBatch()
-    for h in hairs_b:
-        batch.add(h, sdf_b)
+    for h, c in zip(hairs_b, colliders_b):
+        batch.add(h, c)
     system = tfx.TressFXSDFCollisionSystem()
     sim = tfx.TressFXSimulation()
     side = torch.cuda.Stream()
@@ -158,19 +162,20 @@ def test_batch_sparse_sdf_build_and_shared_sdf(gpu):
     prev, changed = None, 0
     for frame in range(8):
         desc = dm.update(0.1 * frame)        # 0.1 s apart: the head turns and the band moves
-        sdf_ref.SetMesh(desc)
-        sdf_b.SetMesh(desc)
+        for c in (sdf_ref, sdf_b, big_ref, big_b):
+            c.SetMesh(desc)
         torch.cuda.synchronize()
         if frame == 5:
             sdf_b.Update(system)
         sdf_ref.Update(system)
+        big_ref.Update(system)
         for sub in range(2):
             for h, w in zip(hairs_ref, wls):
                 w.substeps = 2
                 set_substep_inputs(h, w, frame, sub)
             sim.Simulate(hairs_ref)
-            for h in hairs_ref:
-                sdf_ref.CollideWithHair(system, h)
+            for h, c in zip(hairs_ref, colliders_ref):
+                c.CollideWithHair(system, h)
         batch.begin_frame()
         for sub in range(2):
             ins = [m.substep_inputs(frame, sub) for m in members]
@@ -180,8 +185,9 @@ def test_batch_sparse_sdf_build_and_shared_sdf(gpu):
         batch.run_frame(side.cuda_stream, use_graph=frame % 2 == 1)
         torch.cuda.synchronize()
         ref = sdf_ref.download()
-        mm = first_mismatch(sdf_b.download(), ref)
-        assert mm is None, f"frame {frame}: batched SDF differs at cell {mm[0]}: {mm[1]!r} vs {mm[2]!r} ({mm[3]} cells)"
+        for name, got, want in (("shared", sdf_b.download(), ref), ("128^3", big_b.download(), big_ref.download())):
+            mm = first_mismatch(got, want)
+            assert mm is None, f"frame {frame}: batched {name} SDF differs at cell {mm[0]}: {mm[1]!r} vs {mm[2]!r}"
         if prev is not None:
             changed += int((prev != ref).sum())
         prev = ref

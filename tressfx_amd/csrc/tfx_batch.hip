@@ -334,7 +334,9 @@ extern "C" int tfx_batch_run_frame(tfx_batch* b, void* stream, int use_graph) {
     // Band pointers of each SDF build (the sparse initialise/finalise), in
     // launch order; a frame that fails to launch leaves the history unknown.
     tfx::SdfJob* jobs = reinterpret_cast<tfx::SdfJob*>(b->host[k]);
-    for (size_t j = 0; j < b->sdf_members.size(); ++j) sdf_band_stage(b->members[b->sdf_members[j]].sdf, jobs[j]);
+    const bool sparse = tfx::sdf_multi_sparse(b->max_cells, (int)b->sdf_members.size());
+    for (size_t j = 0; j < b->sdf_members.size(); ++j)
+        sdf_band_stage(b->members[b->sdf_members[j]].sdf, jobs[j], sparse);
     auto failed = [&](int code) {
         for (int j : b->sdf_members) b->members[j].sdf->band_valid = 0;
         b->slot = -1;

@@ -383,7 +383,15 @@ struct SdfJob {
     // and finalised.
     uint8_t* band_cur;
     const uint8_t* band_prev;
+    uint8_t* band_next;   // the band the next build writes: cleared at the end of this one
 };
+
+// Whether a batched build of njobs grids of at most max_cells nodes keeps the
+// band bookkeeping: it saves writing and un-flipping whole grids, at the cost
+// of one more launch, which pays once the grids hold 2^16 bricks in all.
+__host__ __device__ inline bool sdf_multi_sparse(int max_cells, int njobs) {
+    return (long long)(max_cells / 64) * njobs >= (1 << 16);
+}
 
 // One mesh of a batched skinning pass (TressFXBoneSkinning.hlsl:57-94).
 struct MeshSkinJob {

@@ -227,17 +227,24 @@ struct tfx_sdf {
     int band_valid = 0;
 };
 
-// The band pointers of the next batched build of s, advancing its history.
-// Called once per build, in build order, right before it is launched.
-inline void sdf_band_stage(tfx_sdf* s, tfx::SdfJob& jb) {
-    if (!s->band) {
-        jb.band_cur = nullptr;
-        jb.band_prev = nullptr;
+// The band pointers of the next batched build of s, advancing its history
+// (sparse: the build keeps the band bookkeeping, see tfx::sdf_multi_sparse;
+// otherwise it rewrites the whole grid and the history is dropped).  Called
+// once per build, in build order, right before it is launched.
+inline void sdf_band_stage(tfx_sdf* s, tfx::SdfJob& jb, bool sparse) {
+    jb.band_cur = nullptr;
+    jb.band_prev = nullptr;
+    jb.band_next = nullptr;
+    if (!s->band || !sparse) {
+        s->band_valid = 0;
         return;
     }
     const size_t nbrick = (size_t)s->bx * s->by * s->bz;
-    jb.band_cur = s->band + (s->band_parity ? nbrick : 0);
-    jb.band_prev = s->band_valid ? s->band + (s->band_parity ? 0 : nbrick) : nullptr;
+    uint8_t* cur = s->band + (s->band_parity ? nbrick : 0);
+    uint8_t* other = s->band + (s->band_parity ? 0 : nbrick);
+    jb.band_cur = cur;
+    jb.band_prev = s->band_valid ? other : nullptr;
+    jb.band_next = other;
     s->band_parity ^= 1;
     s->band_valid = 1;
 }

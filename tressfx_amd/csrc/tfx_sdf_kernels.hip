@@ -426,20 +426,14 @@ __device__ __forceinline__ int node_brick(int i, const SdfGrid& g, float inv_nx,
 // it already holds if it was outside the previous build's band as well.
 // Initialize therefore writes FloatFlip3(1e10) into this build's band, resets
 // the previous band's other bricks to 1e10 and leaves the rest; Finalize
-// un-flips this build's band only.  The grid equals the full build's.
+// un-flips this build's band only.  The grid equals the full build's.  Marks
+// beyond the touched bricks cost work, never results, so a band only has to
+// cover them.
 
-// Step 1: clear this build's band and the occupancy map.
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_band_clear_multi(const SdfJob* __restrict__ jobs) {
-    const SdfJob& j = jobs[blockIdx.y];
-    const int nbrick = j.g.bx * j.g.by * j.g.bz;
-    uint8_t* bricks = const_cast<uint8_t*>(j.g.bricks);
-    for (int i = blockIdx.x * kSdfBlock + threadIdx.x; i < nbrick; i += gridDim.x * kSdfBlock) {
-        if (bricks) bricks[i] = 0;
-        if (j.band_cur) j.band_cur[i] = 0;
-    }
-}
-
-// Step 2: every brick a triangle's node box overlaps (the box Construct walks).
+// Step 1: every brick a triangle's node box overlaps (the box Construct walks).
+// The band was cleared at the end of the build before (a band that was not,
+// after a build without bookkeeping, only holds extra marks: those bricks are
+// initialised and finalised to 1e10, which they hold anyway).
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_band_mark_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
     if (!j.band_cur) return;
@@ -456,29 +450,42 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_band_mark_multi(const SdfJob*
 }
 
 // Step 3, InitializeSignedDistanceField :301-312 over the band (or the grid).
-// With quads (nx % 4 == 0) a lane takes one brick: its 16 quads (4 nodes along
-// x, one per y and z of the brick), so neighbouring lanes write neighbouring
-// quads and clean bricks cost one byte read.
+// With quads (nx % 4 == 0) kL lanes take one brick, each 16 / kL of its quads
+// (4 nodes along x, one per y and z of the brick): neighbouring lanes write
+// neighbouring quads and clean bricks cost one byte read.  kL trades lanes
+// for work per lane: many grids fill the GPU with one lane per brick, one
+// grid needs more.
+template <int kL>
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_init_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
     const int lane = blockIdx.x * kSdfBlock + threadIdx.x, stride = gridDim.x * kSdfBlock;
+    const SdfGrid& g = j.g;
+    const int nbrick = g.bx * g.by * g.bz;
+    uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
+    if (!j.band_cur || (g.nx & 3) != 0) {
+        // the occupancy map is rebuilt by Finalize
+        if (bricks)
+            for (int i = lane; i < nbrick; i += stride) bricks[i] = 0;
+    }
     if (!j.band_cur) {
         init_cells(j.cells, j.ncell, lane, stride);
         return;
     }
-    const SdfGrid& g = j.g;
     const uint32_t key = float_flip3(1e10f), flat = __float_as_uint(1e10f);
     if ((g.nx & 3) == 0) {
-        const int nbrick = g.bx * g.by * g.bz, nq = g.nx >> 2;
+        const int nq = g.nx >> 2;
         uint4* p = reinterpret_cast<uint4*>(j.cells);
-        for (int b = lane; b < nbrick; b += stride) {
+        for (int w = lane; w < nbrick * kL; w += stride) {
+            const int b = w / kL, part = w % kL;
+            if (bricks && part == 0) bricks[b] = 0;
             const bool cur = j.band_cur[b] != 0;
             if (!cur && j.band_prev && !j.band_prev[b]) continue;
             const uint32_t v = cur ? key : flat;
             const int bx = b % g.bx, r = b / g.bx, by = r % g.by, bz = r / g.by;
-            for (int z = bz << kBrickShift; z < min((bz + 1) << kBrickShift, g.nz); ++z)
-                for (int y = by << kBrickShift; y < min((by + 1) << kBrickShift, g.ny); ++y)
-                    stg(&p[(z * g.ny + y) * nq + bx], make_uint4(v, v, v, v));
+            for (int k = part; k < 16; k += kL) {
+                const int y = (by << kBrickShift) + (k & 3), z = (bz << kBrickShift) + (k >> 2);
+                if (y < g.ny && z < g.nz) stg(&p[(z * g.ny + y) * nq + bx], make_uint4(v, v, v, v));
+            }
         }
     } else {
         const bool fast = j.ncell <= (1 << 24);
@@ -500,12 +507,15 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob*
         construct_triangle<kTeam>(j.cells, j.verts, j.idx, tri, g, lane);
 }
 
-// Finalize of the quads of brick (bx, by, bz), nx % 4 == 0: the same
+// Finalize of quads part, part + kL, ... of brick (bx, by, bz) (the brick's
+// quad k has y = 4 by + k % 4, z = 4 bz + k / 4), nx % 4 == 0: the same
 // un-flip and the same occupancy marks as finalize_quad on each quad.  A
 // touched node marks the bricks its kBrickHalo-dilated extent meets, all within
 // one brick of its own, so the marks are gathered as a 3x3x3 mask (bit
-// 9 (dz+1) + 3 (dy+1) + (dx+1)) and each marked brick is stored once.
-__device__ __forceinline__ void finalize_brick(uint32_t* __restrict__ cells, const SdfGrid& g, int bx, int by, int bz) {
+// 9 (dz+1) + 3 (dy+1) + (dx+1)) and each marked brick is stored once per lane.
+template <int kL>
+__device__ __forceinline__ void finalize_brick(uint32_t* __restrict__ cells, const SdfGrid& g, int bx, int by, int bz,
+                                               int part) {
     const int nq = g.nx >> 2;
     uint4* p = reinterpret_cast<uint4*>(cells);
     const uint32_t far = float_flip3(1e10f);
@@ -528,19 +538,28 @@ __device__ __forceinline__ void finalize_brick(uint32_t* __restrict__ cells, con
             for (int dy = 0; dy < 3; ++dy)
                 if (((zm >> dz) & 1u) && ((ym >> dy) & 1u)) marks |= xm << (9 * dz + 3 * dy);
     };
-    const int y0 = by << kBrickShift, z0 = bz << kBrickShift;
-    if (y0 + 4 <= g.ny && z0 + 4 <= g.nz) {
-        // two rounds of eight loads in flight
-        for (int h = 0; h < 2; ++h) {
-            uint4 u[8];
+    const int ybase = by << kBrickShift, zbase = bz << kBrickShift;
+    constexpr int kQ = 16 / kL;           // quads per lane
+    constexpr int kRound = kQ < 8 ? kQ : 8; // loads in flight together
+    if (ybase + 4 <= g.ny && zbase + 4 <= g.nz) {
+        for (int r0 = 0; r0 < kQ; r0 += kRound) {
+            uint4 u[kRound];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) u[k] = ldg(&p[((z0 + 2 * h + (k >> 2)) * g.ny + y0 + (k & 3)) * nq + bx]);
+            for (int i = 0; i < kRound; ++i) {
+                const int k = part + kL * (r0 + i);
+                u[i] = ldg(&p[((zbase + (k >> 2)) * g.ny + ybase + (k & 3)) * nq + bx]);
+            }
 #pragma unroll
-            for (int k = 0; k < 8; ++k) quad(y0 + (k & 3), z0 + 2 * h + (k >> 2), u[k]);
+            for (int i = 0; i < kRound; ++i) {
+                const int k = part + kL * (r0 + i);
+                quad(ybase + (k & 3), zbase + (k >> 2), u[i]);
+            }
         }
     } else {
-        for (int z = z0; z < min(z0 + 4, g.nz); ++z)
-            for (int y = y0; y < min(y0 + 4, g.ny); ++y) quad(y, z, ldg(&p[(z * g.ny + y) * nq + bx]));
+        for (int k = part; k < 16; k += kL) {
+            const int y = ybase + (k & 3), z = zbase + (k >> 2);
+            if (y < g.ny && z < g.nz) quad(y, z, ldg(&p[(z * g.ny + y) * nq + bx]));
+        }
     }
     if (!marks || !g.bricks) return;
     uint8_t* bricks = const_cast<uint8_t*>(g.bricks);
@@ -549,8 +568,9 @@ __device__ __forceinline__ void finalize_brick(uint32_t* __restrict__ cells, con
             bricks[((bz + k / 9 - 1) * g.by + (by + (k / 3) % 3 - 1)) * g.bx + (bx + k % 3 - 1)] = 1;
 }
 
-// FinalizeSignedDistanceField :400-412 over the band (or the grid), a brick
-// per lane as in the initialisation.
+// FinalizeSignedDistanceField :400-412 over the band (or the grid), kL lanes
+// per brick as in the initialisation.
+template <int kL>
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
     const int lane = blockIdx.x * kSdfBlock + threadIdx.x, stride = gridDim.x * kSdfBlock;
@@ -561,10 +581,11 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* 
     const SdfGrid& g = j.g;
     if ((g.nx & 3) == 0) {
         const int nbrick = g.bx * g.by * g.bz;
-        for (int b = lane; b < nbrick; b += stride) {
+        for (int w = lane; w < nbrick * kL; w += stride) {
+            const int b = w / kL;
             if (!j.band_cur[b]) continue;
             const int bx = b % g.bx, r = b / g.bx;
-            finalize_brick(j.cells, g, bx, r % g.by, r / g.by);
+            finalize_brick<kL>(j.cells, g, bx, r % g.by, r / g.by, w % kL);
         }
     } else {
         const bool fast = j.ncell <= (1 << 24);
@@ -574,12 +595,17 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_finalize_multi(const SdfJob* 
     }
 }
 
+// Occupancy bits, and the next build's band cleared (this build has read it
+// as its previous band).
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_multi(const SdfJob* __restrict__ jobs) {
     const SdfJob& j = jobs[blockIdx.y];
     const int nbrick = j.g.bx * j.g.by * j.g.bz;
     uint32_t* bits = const_cast<uint32_t*>(j.g.brick_bits);
-    for (int w = blockIdx.x * kSdfBlock + threadIdx.x; w * 32 < nbrick; w += gridDim.x * kSdfBlock)
-        bits[w] = pack_brick_word(j.g.bricks, nbrick, w);
+    for (int w = blockIdx.x * kSdfBlock + threadIdx.x; w * 32 < nbrick; w += gridDim.x * kSdfBlock) {
+        if (bits && j.g.bricks) bits[w] = pack_brick_word(j.g.bricks, nbrick, w);
+        if (j.band_next)
+            for (int k = w * 32; k < min(w * 32 + 32, nbrick); ++k) j.band_next[k] = 0;
+    }
 }
 
 }  // namespace tfx
@@ -591,25 +617,37 @@ extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, i
     using namespace tfx;
     if (njobs <= 0) return hipSuccess;
     const int cx = std::max(1, std::min((max_cells / 4 + kSdfBlock - 1) / kSdfBlock, 1024));
-    // a brick per lane: band bookkeeping, and init / finalize over a band (a job
-    // without band bookkeeping covers its whole grid in the same launch through
-    // the grid-stride loops)
-    const int bxg = std::max(1, std::min((max_cells / 64 + kSdfBlock - 1) / kSdfBlock, 1024));
+    // With band bookkeeping (sdf_multi_sparse, the same choice the host made
+    // when it staged the jobs): a brick per lane for the marks, and kL lanes
+    // per brick for init / finalize, one if the grids hold 2^18 bricks in all,
+    // else four.
+    const bool sparse = sdf_multi_sparse(max_cells, njobs);
+    const int max_bricks = std::max(1, max_cells / 64);
+    const int lpb = (long long)max_bricks * njobs >= (1 << 18) ? 1 : 4;
+    const int lxg = std::max(1, std::min((max_bricks * lpb + kSdfBlock - 1) / kSdfBlock, 1024));
+    auto band_pass = [&](void (*k1)(const SdfJob*), void (*k4)(const SdfJob*)) {
+        hipLaunchKernelGGL(lpb == 1 ? k1 : k4, dim3(lxg, njobs), dim3(kSdfBlock), 0, stream, jobs);
+    };
     switch (phase) {
-        case 0: {
-            const int txg = std::max(1, std::min((max_tris + kSdfBlock - 1) / kSdfBlock, 1024));
-            hipLaunchKernelGGL(k_sdf_band_clear_multi, dim3(bxg, njobs), dim3(kSdfBlock), 0, stream, jobs);
-            hipLaunchKernelGGL(k_sdf_band_mark_multi, dim3(txg, njobs), dim3(kSdfBlock), 0, stream, jobs);
-            hipLaunchKernelGGL(k_sdf_init_multi, dim3(bxg, njobs), dim3(kSdfBlock), 0, stream, jobs);
+        case 0:
+            if (sparse) {
+                const int txg = std::max(1, std::min((max_tris + kSdfBlock - 1) / kSdfBlock, 1024));
+                hipLaunchKernelGGL(k_sdf_band_mark_multi, dim3(txg, njobs), dim3(kSdfBlock), 0, stream, jobs);
+                band_pass(k_sdf_init_multi<1>, k_sdf_init_multi<4>);
+            } else {
+                hipLaunchKernelGGL(k_sdf_init_multi<1>, dim3(cx, njobs), dim3(kSdfBlock), 0, stream, jobs);
+            }
             break;
-        }
         case 1: {
             constexpr int kTeam = 16;
             const int tx = std::max(1, std::min((max_tris * kTeam + kSdfBlock - 1) / kSdfBlock, 4096));
             hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kSdfBlock), 0, stream, jobs);
             break;
         }
-        case 2: hipLaunchKernelGGL(k_sdf_finalize_multi, dim3(bxg, njobs), dim3(kSdfBlock), 0, stream, jobs); break;
+        case 2:
+            if (sparse) band_pass(k_sdf_finalize_multi<1>, k_sdf_finalize_multi<4>);
+            else hipLaunchKernelGGL(k_sdf_finalize_multi<1>, dim3(cx, njobs), dim3(kSdfBlock), 0, stream, jobs);
+            break;
         case 3: hipLaunchKernelGGL(k_sdf_pack_multi, dim3(std::max(1, cx / 32), njobs), dim3(kSdfBlock), 0, stream, jobs);
             break;
         default: return hipErrorInvalidValue;

@@ -330,10 +330,11 @@ int launch_pass(tfx_server* s, Pass& pass) {
         jb.ntri = (sd->mesh.vertices && sd->mesh.indices) ? sd->mesh.num_triangles : 0;
         jb.ncell = sd->nx * sd->ny * sd->nz;
         jb.g = sdf_grid(sd);
-        sdf_band_stage(sd, jb);
         max_cells = std::max(max_cells, jb.ncell);
         max_tris = std::max(max_tris, jb.ntri);
     }
+    const bool sparse = tfx::sdf_multi_sparse(max_cells, (int)sdf_members.size());
+    for (size_t j = 0; j < sdf_members.size(); ++j) sdf_band_stage(pass.frames[sdf_members[j]].second->sdf, sjobs[j], sparse);
     // ---- substeps: parameters and bones of every member (UpdateBoneMatrices,
     // UpdateSimulationParameters, then IncreaseSimulationFrame)
     for (int sub = 0; sub < subs; ++sub) {
