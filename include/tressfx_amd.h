@@ -371,7 +371,10 @@ int tfx_sdf_set_collide_variant(tfx_sdf* sdf, int variant);
 /* CollideWithHair, TressFXSDFCollision.cpp:171-221. */
 int tfx_sdf_collide_with_hair(tfx_sdf* sdf, tfx_sdf_collision_system* sys, void* stream, tfx_hair* hair);
 int tfx_sdf_get_grid(const tfx_sdf* sdf, tfx_sdf_grid_info* out);  /* grid getters, .h:76-93 */
-/* GetSDFDataGPUBuffer, .h:71-72: device float[X*Y*Z] (bits as written by Finalize). */
+/* GetSDFDataGPUBuffer, .h:71-72: device float[X*Y*Z] (bits as written by Finalize).
+ * The grid is read-only to the caller: the batched build relies on its
+ * previous contents, so a caller that writes it must tfx_sdf_update before the
+ * next batched frame. */
 int tfx_sdf_get_data_buffer(tfx_sdf* sdf, float** device_ptr);
 int tfx_sdf_download(tfx_sdf* sdf, void* stream, float* host_cells /* X*Y*Z */);
 /* The constant block CollideWithHair would upload for `hair` (TressFXSDFCollision.cpp:184-198). */
