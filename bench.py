@@ -17,7 +17,8 @@ the roofline of the dominant kernel (the fused substep; its duration from HIP
 events around each launch in a second, directly launched leg), the plain
 per-instance dispatch for comparison, the 256-instance batched workload
 against its plain dispatch, the served form (continuous batching over arriving
-instances and step requests) and the CPU transcription's throughput.
+instances and step requests; and its capacity, every instance's frames queued
+at once and drained) and the CPU transcription's throughput.
 """
 import argparse
 import json
@@ -207,6 +208,8 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=12.0,
                     help="length of the served-form request window (0: skip)")
     ap.add_argument("--served-initial", type=int, default=256, help="instances resident when the window opens")
+    ap.add_argument("--served-burst-frames", type=int, default=24,
+                    help="served capacity: frames queued at once on every config-3 instance (0: skip)")
     ap.add_argument("--fur-steps", type=int, default=10, help="timed frames of the config-5 fur workload")
     ap.add_argument("--fur-plain-steps", type=int, default=3)
     ap.add_argument("--fur-served-seconds", type=float, default=8.0,
@@ -371,6 +374,22 @@ def main():
             "resident_at_end": res["resident_at_end"],
             "speedup_vs_plain": (sv / batched["plain"]["value"]) if batched and batched.get("plain") else None,
         }
+        if args.served_burst_frames > 0:
+            srv = tfx.HairServer(hbm_budget_bytes=200 << 30, max_instances=1000, substeps_per_frame=2,
+                                 substep_dt=float(np.float32(1.0 / synth.FRAME_RATE / 2)), max_inflight=2)
+            brun = harness.ServedRun(srv, wls3, initial=0, duration_s=0.0, inst_rate=0.0, req_rate=0.0)
+            _log(f"served capacity: {len(wls3)} instances x {args.served_burst_frames} frames queued at once")
+            bres = brun.burst(args.served_burst_frames)
+            srv.close()
+            bv = bres["vertex_substeps"] / bres["window_s"]
+            served["capacity"] = {
+                "workload": f"the {len(wls3)} config-3 instances resident, {args.served_burst_frames} frames of each "
+                            "submitted at once and drained (requests outrun the GPU: every pass coalesces one "
+                            "frame of every instance)",
+                "value": bv, "unit": "vertex-substeps/s", "window_s": bres["window_s"], "passes": bres["passes"],
+                "mean_instances_per_pass": bres["mean_instances_per_pass"],
+                "speedup_vs_plain": (bv / batched["plain"]["value"]) if batched and batched.get("plain") else None,
+            }
 
     fur = None
     if wl5 is not None:

@@ -365,6 +365,33 @@ class ServedRun:
     def _admit(self, w):
         return self._admit_wl(self.wls[w], self.meshes[w])
 
+    def burst(self, frames):
+        """The served form's capacity: every workload admitted once, `frames`
+        frames of each submitted at once (requests outrun the GPU, so every
+        pass coalesces one frame of every instance), then drained.  Returns
+        the server statistics over the drain and its wall time."""
+        import time
+        srv = self.server
+        ids = []
+        for w in range(len(self.wls)):
+            iid = self._admit(w)
+            if iid is None:
+                raise RuntimeError("the burst's instances do not fit the admission budget")
+            ids.append((iid, w))
+        srv.drain()
+        srv.stats(reset=True)
+        t0 = time.perf_counter()
+        for iid, w in ids:
+            hb, mb, wind = self.inputs[w]
+            srv.submit(iid, hb[:frames], mb[:frames], wind[:frames], track=False)
+        t_sub = time.perf_counter() - t0
+        srv.drain()
+        t1 = time.perf_counter() - t0
+        for iid, _ in ids:
+            srv.retire(iid)
+        return {"window_s": t1, "submit_s": t_sub, "instances": len(ids), "frames_per_instance": int(frames),
+                **srv.stats()}
+
     def run(self, log=None):
         import time
         srv = self.server
