@@ -18,7 +18,11 @@ events around each launch in a second, directly launched leg), the plain
 per-instance dispatch for comparison, the 256-instance batched workload
 against its plain dispatch, the served form (continuous batching over arriving
 instances and step requests; and its capacity, every instance's frames queued
-at once and drained) and the CPU transcription's throughput.
+at once and drained) and the CPU transcription's throughput.  The headline and
+every leg use correctly rounded arithmetic (bit-exact with the FP32
+transcription); `fast_arithmetic` entries repeat the config-2 and config-3
+legs with TFX_ARITH_FAST (hardware division and square root, within the FP64
+tolerance).
 """
 import argparse
 import json
@@ -205,6 +209,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=20, help="timed frames of the 256-instance workload")
     ap.add_argument("--batched-plain-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-fast", action="store_true", help="skip the TFX_ARITH_FAST legs")
     ap.add_argument("--served-seconds", type=float, default=12.0,
                     help="length of the served-form request window (0: skip)")
     ap.add_argument("--served-initial", type=int, default=256, help="instances resident when the window opens")
@@ -313,6 +318,33 @@ def main():
         _log(f"plain leg {plain['ms_per_step']:.3f} ms/frame")
         del pdrv
 
+    # The same legs with TFX_ARITH_FAST (hardware division and square root in
+    # the simulation and collision kernels, within the FP64 tolerance).
+    fast = None
+    if not args.no_fast:
+        tfx.set_arithmetic(tfx.TFX_ARITH_FAST)
+        try:
+            fdrv = harness.BatchDriver(torch, [wl], frames)
+            with torch.cuda.stream(stream):
+                for f in range(W):
+                    fdrv.frame(f, stream.cuda_stream, use_graph=True)
+                fel, _ = _timed_frames(torch, stream, fdrv, frames[W:], use_graph=True)
+            del fdrv
+            fast = {"value": N * wl.substeps * K / fel, "ms_per_step": fel * 1e3 / K, "steps": K}
+            if args.plain_steps > 0:
+                pdrv = harness.FrameDriver(torch, wl, pframes, fused=False)
+                with torch.cuda.stream(stream):
+                    for f in range(W):
+                        pdrv.frame(f, stream.cuda_stream)
+                    pel, _ = _timed_frames(torch, stream, pdrv, pframes[W:])
+                del pdrv
+                fast["plain"] = {"value": N * wl.substeps * args.plain_steps / pel,
+                                 "ms_per_step": pel * 1e3 / args.plain_steps}
+                fast["speedup_vs_plain"] = fast["value"] / plain["value"]
+            _log(f"fast-arithmetic graph leg {fast['ms_per_step']:.3f} ms/frame")
+        finally:
+            tfx.set_arithmetic(tfx.TFX_ARITH_IEEE)
+
     batched = None
     if wls3 is not None:
         n3 = sum(w.asset.num_total_vertices for w in wls3)
@@ -348,6 +380,30 @@ def main():
             "plain": {"value": pval, "steps": P3} if pval else None,
             "speedup_vs_plain": (bval / pval) if pval else None,
         }
+        if not args.no_fast:
+            tfx.set_arithmetic(tfx.TFX_ARITH_FAST)
+            try:
+                bdrv = harness.BatchDriver(torch, wls3, f3)
+                with torch.cuda.stream(stream):
+                    for f in range(W3):
+                        bdrv.frame(f, stream.cuda_stream, use_graph=True)
+                    fel, fms = _timed_frames(torch, stream, bdrv, f3[W3:], use_graph=True)
+                del bdrv
+                fb = {"value": n3 * 2 * K3 / fel, "ms_per_step": fel * 1e3 / K3, "steps": K3,
+                      "latency_ms": {"p50": float(np.percentile(fms, 50)), "p99": float(np.percentile(fms, 99))},
+                      "speedup_vs_plain": (n3 * 2 * K3 / fel / pval) if pval else None}
+                if P3 > 0:
+                    pdrv = harness.PlainMultiDriver(torch, wls3, pf3)
+                    with torch.cuda.stream(stream):
+                        pdrv.frame(0, stream.cuda_stream)
+                        pel, _ = _timed_frames(torch, stream, pdrv, pf3[1:])
+                    del pdrv
+                    fb["plain"] = {"value": n3 * 2 * P3 / pel, "steps": P3}
+                    fb["speedup_vs_fast_plain"] = fb["value"] / fb["plain"]["value"]
+                batched["fast_arithmetic"] = fb
+                _log(f"fast-arithmetic batched leg {fb['ms_per_step']:.3f} ms/frame")
+            finally:
+                tfx.set_arithmetic(tfx.TFX_ARITH_IEEE)
 
     served = None
     if wls3 is not None and args.served_seconds > 0:
@@ -442,6 +498,7 @@ def main():
         "sdf_build_ms_per_frame": sdf_ms / K,
         "graph_equals_direct": graph_equals_direct,
         "plain": plain,
+        "fast_arithmetic": fast,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "batched": batched,
         "served": served,

@@ -380,6 +380,22 @@ int tfx_sdf_download(tfx_sdf* sdf, void* stream, float* host_cells /* X*Y*Z */);
 /* The constant block CollideWithHair would upload for `hair` (TressFXSDFCollision.cpp:184-198). */
 int tfx_sdf_get_collision_params(const tfx_sdf* sdf, const tfx_hair* hair, tfx_sdf_collision_params* out);
 
+/* Arithmetic of the kernels.  No counterpart in the reference, whose HLSL
+ * leaves the precision of division and square root to the driver.
+ * TFX_ARITH_IEEE (default): correctly rounded float division and square root;
+ *   every kernel equals the FP32 transcription bit for bit.
+ * TFX_ARITH_FAST: the hardware reciprocal and square root (a few ulp) in the
+ *   simulation stages, the fused substep and the collision lookups; results
+ *   stay within the stepwise FP64 tolerance (1e-5 relative per substep).  The
+ *   SDF build keeps correctly rounded arithmetic: its grid is the same in
+ *   both modes.
+ * Process-wide; applies to launches issued after the call (a batch re-captures
+ * its frame graph). */
+#define TFX_ARITH_IEEE 0
+#define TFX_ARITH_FAST 1
+int tfx_set_arithmetic(int mode);
+int tfx_get_arithmetic(int* mode);
+
 /* ------------------------------------------------------------------------ */
 /* Asset loading (TressFXAsset, src/TressFX/TressFXAsset.h:48-120)           */
 /* The reference's order: LoadHairData, GenerateFollowHairs, ProcessAsset,   */

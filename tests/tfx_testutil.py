@@ -40,7 +40,7 @@ def perturbed_state(st, seed):
     return out
 
 
-def conditioned_tolerance(asset, st0, advance, seed=0):
+def conditioned_tolerance(asset, st0, advance, seed=0, cond_factor=COND_FACTOR):
     """Per-vertex tolerance against the FP64 oracle for one substep.
 
     `advance(hair64)` runs the substep on an oracle.OracleHair(float64).  The
@@ -58,7 +58,7 @@ def conditioned_tolerance(asset, st0, advance, seed=0):
     pert.set_state(perturbed_state(st0, seed))
     advance(pert)
     sens = {k: rel_err(getattr(pert, k), getattr(ref, k)) for k in ("pos", "prev")}
-    tol = {k: np.maximum(REL_TOL, COND_FACTOR * sens[k]) for k in sens}
+    tol = {k: np.maximum(REL_TOL, cond_factor * sens[k]) for k in sens}
     return ref, tol
 
 

@@ -280,6 +280,8 @@ _I = ctypes.c_int
 _F = ctypes.c_float
 TFX_SDF_COLLIDE_FORWARD = 0
 TFX_SDF_COLLIDE_MIXED = 1
+TFX_ARITH_IEEE = 0
+TFX_ARITH_FAST = 1
 
 EXPORTS = {
     "tfx_last_error": (ctypes.c_char_p, []),
@@ -299,6 +301,8 @@ EXPORTS = {
     "tfx_hair_get_simulation_params": (_I, [_VP, ctypes.POINTER(SimulationParams), c_float_p]),
     "tfx_hair_get_dynamic_state": (_I, [_VP, ctypes.POINTER(DynamicState)]),
     "tfx_sdf_set_collide_variant": (_I, [_VP, _I]),
+    "tfx_set_arithmetic": (_I, [_I]),
+    "tfx_get_arithmetic": (_I, [_VP]),
     "tfx_hair_generate_transforms": (_I, [_VP, _VP, _VP, _VP, _I, _VP, _VP]),
     "tfx_hair_transition_sim_to_rendering": (_I, [_VP, _VP, _VP]),
     "tfx_hair_transition_rendering_to_sim": (_I, [_VP, _VP, _VP]),
@@ -406,6 +410,20 @@ def init(device=0):
 
 def profiling_enable(on=True):
     _check(load_library().tfx_profiling_enable(1 if on else 0))
+
+
+def set_arithmetic(mode):
+    """TFX_ARITH_IEEE (default: correctly rounded division and square root,
+    bit-exact with the FP32 transcription) or TFX_ARITH_FAST (the hardware
+    reciprocal and square root, within the FP64 tolerance), for every launch
+    issued afterwards."""
+    _check(load_library().tfx_set_arithmetic(int(mode)))
+
+
+def get_arithmetic():
+    m = ctypes.c_int()
+    _check(load_library().tfx_get_arithmetic(ctypes.byref(m)))
+    return m.value
 
 
 def rendering_lod(settings, distance, shadow_update=False):

@@ -22,6 +22,16 @@ SOURCES = [
     "tfx_asset.hip",
 ]
 
+# Kernel sources built a second time with the hardware reciprocal and square
+# root (TFX_ARITH_FAST): their kernels go to namespace tfx::fast and their C
+# entry points get a _fast suffix (tfx_device.h).
+FAST_SOURCES = [
+    "tfx_sim_kernels.hip",
+    "tfx_fused_kernel.hip",
+    "tfx_sdf_kernels.hip",
+]
+FAST_FLAGS = ["-DTFX_FAST_BUILD", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: every multiply-add is evaluated as written (the parity
@@ -68,15 +78,36 @@ def build_cpp_test(verbose=False, out=CPP_TEST_BIN):
 
 
 def build_library(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    """Every source to an object (the FAST_SOURCES twice), in parallel, then
+    one link."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+    compile_flags = [f for f in FLAGS if f != "-shared"]
     tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout)
-        raise RuntimeError("hipcc failed building libtressfx_amd.so")
+    with tempfile.TemporaryDirectory() as work:
+        jobs = [(src, os.path.join(work, os.path.splitext(src)[0] + ".o"), []) for src in SOURCES]
+        jobs += [(src, os.path.join(work, os.path.splitext(src)[0] + "_fast.o"), FAST_FLAGS) for src in FAST_SOURCES]
+
+        def compile_one(job):
+            src, obj, extra = job
+            cmd = [HIPCC] + compile_flags + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            return src, subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+        workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 4)))
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            for src, res in pool.map(compile_one, jobs):
+                if res.returncode != 0:
+                    sys.stderr.write(res.stdout)
+                    raise RuntimeError(f"hipcc failed compiling {src}")
+        cmd = [HIPCC] + FLAGS + [obj for _, obj, _ in jobs] + ["-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout)
+            raise RuntimeError("hipcc failed linking libtressfx_amd.so")
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

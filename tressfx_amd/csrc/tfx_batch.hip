@@ -53,6 +53,7 @@ struct tfx_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int exec_substeps[2] = {0, 0};     // substeps captured in exec[k]
     int exec_tris[2] = {0, 0};         // SDF construct grid (largest mesh) captured in exec[k]
+    int exec_arith[2] = {0, 0};        // kernel build (TFX_ARITH_*) captured in exec[k]
 };
 
 static void release_device(tfx_batch* b) {
@@ -145,7 +146,7 @@ static int prepare(tfx_batch* b) {
         const Shape sh = choose_shape(&b->shape, V, hairs.data(), n, cus);
         b->vvariant[V] = sh.variant;
         b->vresident[V] = sh.resident;
-        const int B = tfx_fused_block_vertices(V, sh.variant) / V;
+        const int B = kl_fused_block_vertices(V, sh.variant) / V;
         // Large objects first: their blocks start early and the tail is made
         // of the small ones.
         std::vector<int> order;
@@ -303,7 +304,7 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
                                         "FinalizeSignedDistanceField", "FinalizeSignedDistanceField"};
         for (int phase = 0; phase < 4; ++phase) {
             PassTimer tm(timers ? kNames[phase] : nullptr, s);
-            HIP_TRY(tfx_launch_sdf_multi(phase, jobs, nsdf, b->max_cells, b->max_tris, s));
+            HIP_TRY(kl_launch_sdf_multi(phase, jobs, nsdf, b->max_cells, b->max_tris, s));
         }
     }
     for (int sub = 0; sub < substeps; ++sub)
@@ -311,7 +312,7 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
             if (!b->vcount[V]) continue;
             const int grid = std::min(b->vcount[V], b->vresident[V]);
             PassTimer tm(timers ? "FusedSubstep" : nullptr, s);
-            HIP_TRY(tfx_launch_fused(V, b->vvariant[V], b->dev_tables + (size_t)sub * n, b->dev_items + b->vbegin[V],
+            HIP_TRY(kl_launch_fused(V, b->vvariant[V], b->dev_tables + (size_t)sub * n, b->dev_items + b->vbegin[V],
                                      b->vcount[V], grid, s));
         }
     return TFX_OK;
@@ -319,7 +320,7 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
 
 // Runs the staged frame: parameter upload, batched SDF build, and the staged
 // substeps.  use_graph: the launches are captured once per (slot, substep
-// count, membership) into a hipGraph and replayed.
+// count, membership, kernel arithmetic) into a hipGraph and replayed.
 extern "C" int tfx_batch_run_frame(tfx_batch* b, void* stream, int use_graph) {
     CHECK_ARG(b, "tfx_batch_run_frame: null batch");
     CHECK_ARG(b->slot >= 0, "tfx_batch_run_frame: no frame staged");
@@ -344,7 +345,9 @@ extern "C" int tfx_batch_run_frame(tfx_batch* b, void* stream, int use_graph) {
     };
     if (use_graph) {
         CHECK_ARG(s != nullptr, "tfx_batch_run_frame: graph replay needs a non-default stream");
-        if (!b->exec[k] || b->exec_substeps[k] != substeps || b->exec_tris[k] != b->max_tris) {
+        const int arith = g_arith;
+        if (!b->exec[k] || b->exec_substeps[k] != substeps || b->exec_tris[k] != b->max_tris ||
+            b->exec_arith[k] != arith) {
             if (b->exec[k]) { (void)hipGraphExecDestroy(b->exec[k]); b->exec[k] = nullptr; }
             hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return failed(fail(TFX_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e)));
@@ -358,6 +361,7 @@ extern "C" int tfx_batch_run_frame(tfx_batch* b, void* stream, int use_graph) {
             if (e != hipSuccess) return failed(fail(TFX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)));
             b->exec_substeps[k] = substeps;
             b->exec_tris[k] = b->max_tris;
+            b->exec_arith[k] = arith;
         }
         PassTimer tm("BatchFrame", s);
         hipError_t e = hipGraphLaunch(b->exec[k], s);

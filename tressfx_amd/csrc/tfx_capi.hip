@@ -129,6 +129,18 @@ extern "C" int tfx_init(int hip_device) {
     return TFX_OK;
 }
 
+extern "C" int tfx_set_arithmetic(int mode) {
+    CHECK_ARG(mode == TFX_ARITH_IEEE || mode == TFX_ARITH_FAST, "tfx_set_arithmetic: unknown mode");
+    g_arith = mode;
+    return TFX_OK;
+}
+
+extern "C" int tfx_get_arithmetic(int* mode) {
+    CHECK_ARG(mode, "tfx_get_arithmetic: null argument");
+    *mode = g_arith;
+    return TFX_OK;
+}
+
 extern "C" void tfx_simulation_settings_default(tfx_simulation_settings* s) {
     if (!s) return;
     s->vsp_coeff = 0.8f;
@@ -405,7 +417,7 @@ extern "C" int tfx_simulation_run_stage(tfx_simulation* sim, void* stream, tfx_h
     if (rc) return rc;
     DevHair d = dev_hair(h, nullptr);
     PassTimer tm(kStageNames[stage], s);
-    HIP_TRY(tfx_launch_stage(stage, &d, s));
+    HIP_TRY(kl_launch_stage(stage, &d, s));
     return TFX_OK;
 }
 
@@ -424,7 +436,7 @@ extern "C" int tfx_simulation_simulate(tfx_simulation* sim, void* stream, tfx_ha
             const int reps = (stage == 3) ? hairs[i]->params.cpu_local_shape_iterations : 1;
             for (int r = 0; r < reps; ++r) {
                 PassTimer tm(kStageNames[stage], s);
-                HIP_TRY(tfx_launch_stage(stage, &d, s));
+                HIP_TRY(kl_launch_stage(stage, &d, s));
             }
         }
     }
@@ -433,7 +445,7 @@ extern "C" int tfx_simulation_simulate(tfx_simulation* sim, void* stream, tfx_ha
 }
 
 extern "C" int tfx_simulation_set_launch_shape(tfx_simulation* sim, int V, int variant, int blocks_per_cu) {
-    CHECK_ARG(sim && tfx_fused_block_vertices(V, 0) > 0, "tfx_simulation_set_launch_shape: bad vertices per strand");
+    CHECK_ARG(sim && kl_fused_block_vertices(V, 0) > 0, "tfx_simulation_set_launch_shape: bad vertices per strand");
     CHECK_ARG(variant >= -1 && variant <= 2, "tfx_simulation_set_launch_shape: variant must be -1 (automatic), 0, 1 or 2");
     CHECK_ARG(blocks_per_cu >= 0 && blocks_per_cu <= 64, "tfx_simulation_set_launch_shape: blocks_per_cu out of range");
     sim->variant[V] = variant;
@@ -478,7 +490,7 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         const Shape sh = choose_shape(sim, V, hairs, n, cus);
         vclass_variant[V] = sh.variant;
         vclass_resident[V] = sh.resident;
-        const int B = tfx_fused_block_vertices(V, sh.variant) / V;
+        const int B = kl_fused_block_vertices(V, sh.variant) / V;
         for (int i = 0; i < n; ++i)
             if (hairs[i]->V == V)
                 for (int g0 = 0; g0 < hairs[i]->G; g0 += B) items.push_back(tfx::WorkItem{i, g0});
@@ -505,7 +517,7 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         if (!cnt) continue;
         const int grid = std::min(cnt, vclass_resident[V]);
         PassTimer tm("FusedSubstep", s);
-        HIP_TRY(tfx_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, s));
+        HIP_TRY(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, s));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
     return TFX_OK;
@@ -633,7 +645,7 @@ extern "C" int tfx_sdf_update(tfx_sdf* s, tfx_sdf_collision_system* sys, void* s
                                     "FinalizeSignedDistanceField", "FinalizeSignedDistanceField"};
     for (int phase = 0; phase < 4; ++phase) {
         PassTimer tm(kNames[phase], (hipStream_t)stream);
-        HIP_TRY(tfx_launch_sdf_phase(phase, s->cells, ncell, s->mesh.vertices, s->mesh.indices, s->mesh.num_triangles,
+        HIP_TRY(kl_launch_sdf_phase(phase, s->cells, ncell, s->mesh.vertices, s->mesh.indices, s->mesh.num_triangles,
                                      sdf_grid(s), (hipStream_t)stream));
     }
     return TFX_OK;
@@ -646,10 +658,10 @@ extern "C" int tfx_sdf_collide_with_hair(tfx_sdf* s, tfx_sdf_collision_system* s
     PassTimer tm("CollideHairVerticesWithSdf", (hipStream_t)stream);
     const float* cells = reinterpret_cast<const float*>(s->cells);
     if (s->collide_variant == TFX_SDF_COLLIDE_MIXED)
-        HIP_TRY(tfx_launch_sdf_collide_mixed(h->pos, h->prev, h->N, cells, sdf_grid(s), s->margin * s->cell,
+        HIP_TRY(kl_launch_sdf_collide_mixed(h->pos, h->prev, h->N, cells, sdf_grid(s), s->margin * s->cell,
                                              (hipStream_t)stream));
     else
-        HIP_TRY(tfx_launch_sdf_collide(h->pos, h->prev, h->N, h->V, cells, sdf_grid(s), s->margin * s->cell,
+        HIP_TRY(kl_launch_sdf_collide(h->pos, h->prev, h->N, h->V, cells, sdf_grid(s), s->margin * s->cell,
                                        (hipStream_t)stream));
     return TFX_OK;
 }
@@ -790,7 +802,7 @@ extern "C" int tfx_hair_generate_transforms(tfx_hair* h, const tfx_hair_to_trian
                                             const float* skinned, int nverts, tfx_strand_transform* out, void* stream) {
     CHECK_ARG(h && map && mesh && skinned && out && nverts >= 0, "tfx_hair_generate_transforms: bad argument");
     HIP_TRY(hipSetDevice(h->device));
-    HIP_TRY(tfx_launch_generate_transforms(h->rest, h->G, h->F, h->V, map, reinterpret_cast<const float4*>(mesh),
+    HIP_TRY(kl_launch_generate_transforms(h->rest, h->G, h->F, h->V, map, reinterpret_cast<const float4*>(mesh),
                                            reinterpret_cast<const float4*>(skinned), nverts, out, (hipStream_t)stream));
     return TFX_OK;
 }

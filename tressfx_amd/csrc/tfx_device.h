@@ -8,12 +8,26 @@
 //   float -> int = truncation, NaN -> 0, saturating (D3D ftoi)
 // All HIP sources are compiled with -ffp-contract=off, so no product is
 // fused into an add behind the source's back.
+//
+// The kernel sources are built twice.  The default build divides and takes
+// square roots correctly rounded; the second (TFX_FAST_BUILD, with hipcc's
+// -fno-hip-fp32-correctly-rounded-divide-sqrt) uses the hardware reciprocal
+// and square root.  That build's kernels live in tfx::fast and its C entry
+// points carry a _fast suffix; the host chooses a build per launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/tressfx_amd.h"
+
+#ifdef TFX_FAST_BUILD
+#define TFX_KERNEL_NS fast
+#define TFX_KSYM(name) name##_fast
+#else
+#define TFX_KERNEL_NS exact
+#define TFX_KSYM(name) name
+#endif
 
 namespace tfx {
 

@@ -22,6 +22,7 @@
 #include "tfx_device.h"
 
 namespace tfx {
+inline namespace TFX_KERNEL_NS {
 
 constexpr int kFusedBlock = 256;
 constexpr int kMaxStagedFollow = 4;  // follow strands per guide whose offsets are staged in LDS
@@ -525,21 +526,22 @@ static hipError_t launch_fused(const DevHair* hairs, const WorkItem* items, int 
     return hipGetLastError();
 }
 
+}  // namespace TFX_KERNEL_NS
 }  // namespace tfx
 
 // Vertices per strand block for each strand length and launch-shape variant:
 // variant 0 = 512, 1 = 1024, 2 = 2048 guide vertices (at most 128 strands).
-extern "C" int tfx_fused_block_vertices(int V, int variant) {
+extern "C" int TFX_KSYM(tfx_fused_block_vertices)(int V, int variant) {
     if (V != 4 && V != 8 && V != 16 && V != 32 && V != 64) return 0;
     if (variant < 0 || variant > 2) return 0;
     const int vb = 512 << variant;
     return vb / V > 128 ? 128 * V : vb;
 }
 
-extern "C" hipError_t tfx_launch_fused(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
+extern "C" hipError_t TFX_KSYM(tfx_launch_fused)(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
                                        int nitems, int grid, hipStream_t stream) {
     using namespace tfx;
-    const int vb = tfx_fused_block_vertices(V, variant);
+    const int vb = TFX_KSYM(tfx_fused_block_vertices)(V, variant);
     switch (V * 10000 + vb) {
         case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, stream);
         case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, stream);
@@ -558,9 +560,9 @@ extern "C" hipError_t tfx_launch_fused(int V, int variant, const tfx::DevHair* h
 }
 
 // Resident blocks per CU of one launch shape (0 if the shape does not exist).
-extern "C" int tfx_fused_occupancy(int V, int variant) {
+extern "C" int TFX_KSYM(tfx_fused_occupancy)(int V, int variant) {
     using namespace tfx;
-    const int vb = tfx_fused_block_vertices(V, variant);
+    const int vb = TFX_KSYM(tfx_fused_block_vertices)(V, variant);
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
 #define TFX_OCC(Vv, VBv) \

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -22,21 +23,47 @@ using tfx::DevHair;
 using tfx::DevParams;
 using tfx::SdfGrid;
 
-extern "C" hipError_t tfx_launch_stage(int stage, const tfx::DevHair* h, hipStream_t stream);
+// The kernel sources are built twice (tfx_device.h): tfx_x is the correctly
+// rounded build's entry point, tfx_x_fast the hardware-division build's.
+// g_arith (TFX_ARITH_*) picks the build for every launch issued after it is
+// set; kl_x forwards to that build.  The SDF build always runs the correctly
+// rounded kernels, so the grid is the same in both modes (hardware division
+// could move a triangle's node box by a node and with it the band).
+inline std::atomic<int> g_arith{TFX_ARITH_IEEE};
+#define TFX_DUAL(ret, name, params, args)                                           \
+    extern "C" ret tfx_##name params;                                                \
+    extern "C" ret tfx_##name##_fast params;                                         \
+    inline ret kl_##name params { return g_arith == TFX_ARITH_FAST ? tfx_##name##_fast args : tfx_##name args; }
+TFX_DUAL(hipError_t, launch_stage, (int stage, const tfx::DevHair* h, hipStream_t stream), (stage, h, stream))
 extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
                                            const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream);
-extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, int V, const float* sdf,
-                                             tfx::SdfGrid g, float margin, hipStream_t stream);
-extern "C" hipError_t tfx_launch_generate_transforms(const float4* rest, int G, int F, int V,
-                                                     const tfx_hair_to_triangle_mapping* map, const float4* mesh,
-                                                     const float4* skinned, int nverts, tfx_strand_transform* out,
-                                                     hipStream_t stream);
-extern "C" hipError_t tfx_launch_sdf_collide_mixed(float4* pos, float4* prev, int n, const float* sdf, tfx::SdfGrid g,
-                                                   float margin, hipStream_t stream);
-extern "C" hipError_t tfx_launch_fused(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
-                                       int nitems, int grid, hipStream_t stream);
-extern "C" int tfx_fused_block_vertices(int V, int variant);
-extern "C" int tfx_fused_occupancy(int V, int variant);
+inline hipError_t kl_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
+                                      const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream) {
+    return tfx_launch_sdf_phase(phase, sdf, ncell, verts, idx, ntri, g, stream);
+}
+TFX_DUAL(hipError_t, launch_sdf_collide,
+         (float4* pos, float4* prev, int n, int V, const float* sdf, tfx::SdfGrid g, float margin, hipStream_t stream),
+         (pos, prev, n, V, sdf, g, margin, stream))
+TFX_DUAL(hipError_t, launch_generate_transforms,
+         (const float4* rest, int G, int F, int V, const tfx_hair_to_triangle_mapping* map, const float4* mesh,
+          const float4* skinned, int nverts, tfx_strand_transform* out, hipStream_t stream),
+         (rest, G, F, V, map, mesh, skinned, nverts, out, stream))
+TFX_DUAL(hipError_t, launch_sdf_collide_mixed,
+         (float4* pos, float4* prev, int n, const float* sdf, tfx::SdfGrid g, float margin, hipStream_t stream),
+         (pos, prev, n, sdf, g, margin, stream))
+TFX_DUAL(hipError_t, launch_fused,
+         (int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, int grid,
+          hipStream_t stream),
+         (V, variant, hairs, items, nitems, grid, stream))
+TFX_DUAL(int, fused_block_vertices, (int V, int variant), (V, variant))
+TFX_DUAL(int, fused_occupancy, (int V, int variant), (V, variant))
+extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
+                                           hipStream_t stream);
+inline hipError_t kl_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
+                                      hipStream_t stream) {
+    return tfx_launch_sdf_multi(phase, jobs, njobs, max_cells, max_tris, stream);
+}
+#undef TFX_DUAL
 
 // ---------------------------------------------------------------------------
 // Errors
@@ -265,8 +292,6 @@ struct tfx_mesh_skinning {
 
 extern "C" hipError_t tfx_launch_mesh_skin_multi(const tfx::MeshSkinJob* jobs, int njobs, int max_vertices,
                                                  hipStream_t stream);
-extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
-                                           hipStream_t stream);
 
 // The bounding box that follows the follow bone (TressFXBoneSkinning::
 // GetBoundingBox, TressFXBoneSkinning.cpp:439-452): the rest cube moved by the
@@ -436,7 +461,7 @@ struct Shape { int variant; int resident; };
 
 inline Shape choose_shape(const tfx_simulation* sim, int V, tfx_hair* const* hairs, int n, int cus) {
     auto resident_of = [&](int variant) {
-        const int occ = tfx_fused_occupancy(V, variant);
+        const int occ = kl_fused_occupancy(V, variant);
         const int per_cu = sim->grid_per_cu > 0 ? std::min(sim->grid_per_cu, std::max(occ, 1)) : std::max(occ, 1);
         return per_cu * cus;
     };
@@ -444,8 +469,8 @@ inline Shape choose_shape(const tfx_simulation* sim, int V, tfx_hair* const* hai
     Shape best{0, resident_of(0)};
     long long best_rounds = -1;
     for (int variant = 0; variant <= 2; ++variant) {
-        const int vb = tfx_fused_block_vertices(V, variant);
-        if (variant > 0 && vb == tfx_fused_block_vertices(V, variant - 1)) continue;
+        const int vb = kl_fused_block_vertices(V, variant);
+        if (variant > 0 && vb == kl_fused_block_vertices(V, variant - 1)) continue;
         const int B = vb / V;
         long long items = 0;
         for (int i = 0; i < n; ++i)

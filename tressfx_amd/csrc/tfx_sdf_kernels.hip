@@ -14,6 +14,7 @@
 #include "tfx_device.h"
 
 namespace tfx {
+inline namespace TFX_KERNEL_NS {
 
 constexpr int kSdfBlock = 256;
 
@@ -345,10 +346,11 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_collide_mixed(float4* __restr
     }
 }
 
+}  // namespace TFX_KERNEL_NS
 }  // namespace tfx
 
 // phase 0 = Initialize, 1 = Construct, 2 = Finalize, 3 = pack the brick occupancy to bits
-extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
+extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_phase)(int phase, uint32_t* sdf, int ncell, const tfx_standard_vertex* verts,
                                            const uint32_t* idx, int ntri, tfx::SdfGrid g, hipStream_t stream) {
     using namespace tfx;
     // init and finalize take four nodes per lane where the grid allows it
@@ -380,7 +382,7 @@ extern "C" hipError_t tfx_launch_sdf_phase(int phase, uint32_t* sdf, int ncell, 
     return hipGetLastError();
 }
 
-extern "C" hipError_t tfx_launch_sdf_collide_mixed(float4* pos, float4* prev, int n, const float* sdf, tfx::SdfGrid g,
+extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_collide_mixed)(float4* pos, float4* prev, int n, const float* sdf, tfx::SdfGrid g,
                                                    float margin, hipStream_t stream) {
     using namespace tfx;
     if (n <= 0) return hipSuccess;
@@ -389,7 +391,7 @@ extern "C" hipError_t tfx_launch_sdf_collide_mixed(float4* pos, float4* prev, in
     return hipGetLastError();
 }
 
-extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, int V, const float* sdf,
+extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_collide)(float4* pos, float4* prev, int n, int V, const float* sdf,
                                              tfx::SdfGrid g, float margin, hipStream_t stream) {
     using namespace tfx;
     if (n <= 0) return hipSuccess;
@@ -404,6 +406,7 @@ extern "C" hipError_t tfx_launch_sdf_collide(float4* pos, float4* prev, int n, i
 // with new meshes and grid origins.
 // ---------------------------------------------------------------------------
 namespace tfx {
+inline namespace TFX_KERNEL_NS {
 
 // Brick of node i of a whole-grid pass.
 __device__ __forceinline__ int node_brick(int i, const SdfGrid& g, float inv_nx, float inv_ny, bool fast) {
@@ -608,11 +611,12 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_multi(const SdfJob* __re
     }
 }
 
+}  // namespace TFX_KERNEL_NS
 }  // namespace tfx
 
 // Batched build of `njobs` SDFs (job table in device memory).  `max_cells` /
 // `max_tris` bound the per-job work and size the grid's x extent.
-extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
+extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_multi)(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
                                            hipStream_t stream) {
     using namespace tfx;
     if (njobs <= 0) return hipSuccess;

@@ -269,7 +269,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
             return pass.frames[a].second->substeps > pass.frames[b].second->substeps;
         });
         shapes[V] = choose_shape(&s->shape, V, hairs.data(), n, cus);
-        const int B = tfx_fused_block_vertices(V, shapes[V].variant) / V;
+        const int B = kl_fused_block_vertices(V, shapes[V].variant) / V;
         for (int m : mem) {
             for (int g0 = 0; g0 < hairs[m]->G; g0 += B) items.push_back(tfx::WorkItem{m, g0});
             const int upto = (int)items.size() - vbegin[V];
@@ -390,7 +390,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
     if (!sdf_members.empty()) {
         const tfx::SdfJob* dj = reinterpret_cast<const tfx::SdfJob*>(D + off_sjobs);
         for (int phase = 0; phase < 4; ++phase)
-            HIP_TRY(tfx_launch_sdf_multi(phase, dj, (int)sdf_members.size(), max_cells, max_tris, st));
+            HIP_TRY(kl_launch_sdf_multi(phase, dj, (int)sdf_members.size(), max_cells, max_tris, st));
     }
     const DevHair* dtables = reinterpret_cast<const DevHair*>(D + off_tables);
     const tfx::WorkItem* ditems = reinterpret_cast<const tfx::WorkItem*>(D + off_items);
@@ -399,7 +399,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
             const int cnt = vcount[(size_t)V * subs + sub];
             if (!cnt) continue;
             const int grid = std::min(cnt, shapes[V].resident);
-            HIP_TRY(tfx_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], cnt, grid,
+            HIP_TRY(kl_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], cnt, grid,
                                      st));
         }
     HIP_TRY(hipEventRecord(pass.ev, st));

@@ -8,6 +8,7 @@
 #include "tfx_device.h"
 
 namespace tfx {
+inline namespace TFX_KERNEL_NS {
 
 constexpr int kBlock = 256;
 
@@ -322,10 +323,11 @@ __global__ void __launch_bounds__(kBlock) k_generate_transforms(const float4* __
     out[s] = o;
 }
 
+}  // namespace TFX_KERNEL_NS
 }  // namespace tfx
 
 // Launchers used by the C-ABI layer.
-extern "C" hipError_t tfx_launch_stage(int stage, const tfx::DevHair* h, hipStream_t stream) {
+extern "C" hipError_t TFX_KSYM(tfx_launch_stage)(int stage, const tfx::DevHair* h, hipStream_t stream) {
     using namespace tfx;
     const int nv = h->G * h->V;
     const int vblocks = (nv + kBlock - 1) / kBlock;
@@ -343,7 +345,7 @@ extern "C" hipError_t tfx_launch_stage(int stage, const tfx::DevHair* h, hipStre
     return hipGetLastError();
 }
 
-extern "C" hipError_t tfx_launch_generate_transforms(const float4* rest, int G, int F, int V,
+extern "C" hipError_t TFX_KSYM(tfx_launch_generate_transforms)(const float4* rest, int G, int F, int V,
                                                      const tfx_hair_to_triangle_mapping* map, const float4* mesh,
                                                      const float4* skinned, int nverts, tfx_strand_transform* out,
                                                      hipStream_t stream) {
