@@ -195,6 +195,14 @@ public:
     TressFXSimulation& operator=(const TressFXSimulation&) = delete;
 
     static void Initialize(int device) { check(tfx_init(device)); }
+    /* Kernel arithmetic (TFX_ARITH_IEEE / TFX_ARITH_FAST), process-wide; no
+     * counterpart in the reference. */
+    static void SetArithmetic(int mode) { check(tfx_set_arithmetic(mode)); }
+    static int GetArithmetic() {
+        int m = 0;
+        check(tfx_get_arithmetic(&m));
+        return m;
+    }
 
     void Simulate(void* stream, std::vector<TressFXHairObject*>& hairObjects) {
         std::vector<tfx_hair*> h;

@@ -192,6 +192,10 @@ int main(int argc, char** argv) {
         REQUIRE(hipStreamSynchronize(render) == hipSuccess);
         REQUIRE(hipStreamDestroy(render) == hipSuccess);
         sdfA.SetCollideVariant(TFX_SDF_COLLIDE_FORWARD);
+        REQUIRE(TressFXSimulation::GetArithmetic() == TFX_ARITH_IEEE);
+        TressFXSimulation::SetArithmetic(TFX_ARITH_FAST);
+        REQUIRE(TressFXSimulation::GetArithmetic() == TFX_ARITH_FAST);
+        TressFXSimulation::SetArithmetic(TFX_ARITH_IEEE);
         tfx_dynamic_state st = hairA.GetDynamicState();
         REQUIRE(st.positions != nullptr);
         check(tfx_hair_download_state(hairA.handle(), nullptr, pa.data(), qa.data(), nullptr, nullptr, nullptr));
