@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--batched-plain-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-fast", action="store_true", help="skip the TFX_ARITH_FAST legs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched legs keep the occupancy heuristic's launch shapes instead of the autotuner's")
     ap.add_argument("--served-seconds", type=float, default=12.0,
                     help="length of the served-form request window (0: skip)")
     ap.add_argument("--served-initial", type=int, default=256, help="instances resident when the window opens")
@@ -351,12 +353,14 @@ def main():
         W3, K3, P3 = 3, args.batched_steps, args.batched_plain_steps
         f3 = list(range(W3 + K3))
         bdrv = harness.BatchDriver(torch, wls3, f3)
+        tune = 2 if not args.no_autotune else 0
         with torch.cuda.stream(stream):
             for f in range(W3):
-                bdrv.frame(f, stream.cuda_stream, use_graph=True)
+                bdrv.frame(f, stream.cuda_stream, use_graph=True, autotune_repeats=tune if f == W3 - 1 else 0)
             bel, bms = _timed_frames(torch, stream, bdrv, f3[W3:], use_graph=True)
         _log(f"batched leg {bel * 1e3 / K3:.3f} ms/frame")
         bval = n3 * 2 * K3 / bel
+        autotune = getattr(bdrv, "autotune_records", None)
         del bdrv
         pval = None
         if P3 > 0:
@@ -379,6 +383,7 @@ def main():
             "latency_ms": {"p50": float(np.percentile(bms, 50)), "p99": float(np.percentile(bms, 99))},
             "plain": {"value": pval, "steps": P3} if pval else None,
             "speedup_vs_plain": (bval / pval) if pval else None,
+            "autotune": autotune,
         }
         if not args.no_fast:
             tfx.set_arithmetic(tfx.TFX_ARITH_FAST)
@@ -386,12 +391,13 @@ def main():
                 bdrv = harness.BatchDriver(torch, wls3, f3)
                 with torch.cuda.stream(stream):
                     for f in range(W3):
-                        bdrv.frame(f, stream.cuda_stream, use_graph=True)
+                        bdrv.frame(f, stream.cuda_stream, use_graph=True, autotune_repeats=tune if f == W3 - 1 else 0)
                     fel, fms = _timed_frames(torch, stream, bdrv, f3[W3:], use_graph=True)
+                fast_tune = getattr(bdrv, "autotune_records", None)
                 del bdrv
                 fb = {"value": n3 * 2 * K3 / fel, "ms_per_step": fel * 1e3 / K3, "steps": K3,
                       "latency_ms": {"p50": float(np.percentile(fms, 50)), "p99": float(np.percentile(fms, 99))},
-                      "speedup_vs_plain": (n3 * 2 * K3 / fel / pval) if pval else None}
+                      "speedup_vs_plain": (n3 * 2 * K3 / fel / pval) if pval else None, "autotune": fast_tune}
                 if P3 > 0:
                     pdrv = harness.PlainMultiDriver(torch, wls3, pf3)
                     with torch.cuda.stream(stream):

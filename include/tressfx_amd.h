@@ -483,6 +483,25 @@ int tfx_batch_remove(tfx_batch* batch, tfx_hair* hair);
 int tfx_batch_count(const tfx_batch* batch, int* num_members);
 /* As tfx_simulation_set_launch_shape, for the batch's fused launches. */
 int tfx_batch_set_launch_shape(tfx_batch* batch, int vertices_per_strand, int variant, int blocks_per_cu);
+/* Autotuner of the batch's launch shapes (no counterpart in the reference).
+ * Call with a frame staged (after tfx_batch_stage_substep, before
+ * tfx_batch_run_frame), best after frames have run so that states and SDFs are
+ * representative.  For every strand-length class, each strand-block size and
+ * each persistent-blocks-per-CU count up to the kernel's occupancy runs the
+ * class's fused launch of the staged first substep `repeats` times from the
+ * members' state (restored after every run: results are unaffected), and the
+ * fastest is pinned for the class until tfx_batch_set_launch_shape changes
+ * it.  One record per class (at most max_out written, *num_out = classes). */
+typedef struct tfx_autotune_record {
+    int vertices_per_strand;
+    int variant;         /* chosen: 0, 1, 2 = strand blocks of 512, 1024, 2048 guide vertices */
+    int blocks_per_cu;   /* chosen persistent blocks per CU */
+    int candidates;      /* shapes timed */
+    float chosen_us;     /* the chosen shape's fastest launch */
+    float default_us;    /* the shape in use before the call (0 if not among the candidates) */
+} tfx_autotune_record;
+int tfx_batch_autotune(tfx_batch* batch, void* stream, int repeats, tfx_autotune_record* out, int max_out,
+                       int* num_out);
 /* Takes a free staging slot and moves every member's SDF grid to its mesh's
  * current bounding box (meshes are set with tfx_sdf_set_mesh beforehand). */
 int tfx_batch_begin_frame(tfx_batch* batch);

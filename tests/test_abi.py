@@ -55,6 +55,7 @@ RECORDS = {
     "tfx_instance_desc": tfx.InstanceDesc,
     "tfx_server_stats": tfx.ServerStats,
     "tfx_rendering_lod_settings": tfx.RenderingLodSettings,
+    "tfx_autotune_record": tfx.AutotuneRecord,
     "tfx_hair_to_triangle_mapping": tfx.HairToTriangleMapping,
     "tfx_strand_transform": tfx.StrandTransform,
 }

@@ -239,3 +239,59 @@ def test_batch_errors(gpu):
     b.run_frame()                                # empty batch: nothing to do
     with pytest.raises(tfx.TFXError):
         b.set_launch_shape(12, 0)
+
+
+def test_batch_autotune_keeps_results(gpu):
+    """The autotuner times every launch shape of each strand length on the
+    staged frame (from a snapshot of the state, restored after every run) and
+    pins the fastest: the frame it was called in, and later graph-replayed
+    frames with the pinned shapes, equal the per-object reference dispatch bit
+    for bit."""
+    torch = gpu
+    print("[batch] autotune, 4 members, 4 frames", flush=True)
+    members = [Member(torch, i, *spec, copies=2) for i, spec in enumerate(MEMBERS[:4])]
+    b = tfx.HairBatch()
+    for m in members:
+        b.add(m.hairs[1], m.sdfs[1])
+    system = tfx.TressFXSDFCollisionSystem()
+    sim = tfx.TressFXSimulation()
+    side = torch.cuda.Stream()
+    records = None
+    for frame in range(4):
+        for m in members:
+            if m.dm is not None:
+                desc = m.dm.update(frame / synth.FRAME_RATE)
+                for s in m.sdfs:
+                    s.SetMesh(desc)
+        torch.cuda.synchronize()
+        for m in members:
+            if m.sdfs[0] is not None:
+                m.sdfs[0].Update(system)
+        for sub in range(2):
+            for m in members:
+                m.wl.substeps = 2
+                set_substep_inputs(m.hairs[0], m.wl, frame, sub)
+            sim.Simulate([m.hairs[0] for m in members])
+            for m in members:
+                if m.sdfs[0] is not None:
+                    m.sdfs[0].CollideWithHair(system, m.hairs[0])
+        b.begin_frame()
+        for sub in range(2):
+            ins = [m.substep_inputs(frame, sub) for m in members]
+            b.stage_substep(np.stack([x[0] for x in ins]), (tfx.SimulationSettings * len(members))(*[x[1] for x in ins]),
+                            np.float32(1.0 / synth.FRAME_RATE / 2))
+        if frame == 1:
+            records = b.autotune(side.cuda_stream, repeats=2)
+        b.run_frame(side.cuda_stream, use_graph=frame >= 1)
+        torch.cuda.synchronize()
+        for m in members:
+            ref, got = m.hairs[0].download_state(), m.hairs[1].download_state()
+            for k in ("pos", "prev", "tangent"):
+                mm = first_mismatch(got[k], ref[k])
+                assert mm is None, f"frame {frame} V={m.wl.asset.num_vertices_per_strand} {k}: {mm}"
+    lengths = sorted({m.wl.asset.num_vertices_per_strand for m in members})
+    assert sorted(r["vertices_per_strand"] for r in records) == lengths
+    for r in records:
+        print(r)
+        assert r["variant"] in (0, 1, 2) and r["blocks_per_cu"] >= 1 and r["candidates"] >= 1
+        assert 0 < r["chosen_us"] and (r["default_us"] == 0 or r["chosen_us"] <= r["default_us"])

@@ -256,6 +256,13 @@ class RenderingLodSettings(ctypes.Structure):
         return s
 
 
+class AutotuneRecord(ctypes.Structure):
+    """tfx_autotune_record: the launch shape the batch autotuner chose for one
+    strand length, its fastest launch and that of the shape in use before."""
+    _fields_ = [("vertices_per_strand", ctypes.c_int), ("variant", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("candidates", ctypes.c_int), ("chosen_us", ctypes.c_float), ("default_us", ctypes.c_float)]
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [
         ("completed_requests", ctypes.c_longlong),
@@ -365,6 +372,7 @@ EXPORTS = {
     "tfx_batch_remove": (_I, [_VP, _VP]),
     "tfx_batch_count": (_I, [_VP, ctypes.POINTER(_I)]),
     "tfx_batch_set_launch_shape": (_I, [_VP, _I, _I, _I]),
+    "tfx_batch_autotune": (_I, [_VP, _VP, _I, _VP, _I, _VP]),
     "tfx_batch_begin_frame": (_I, [_VP]),
     "tfx_batch_stage_substep": (_I, [_VP, c_float_p, _I, ctypes.POINTER(SimulationSettings), _I, _F, c_float_p]),
     "tfx_batch_run_frame": (_I, [_VP, _VP, _I]),
@@ -809,6 +817,15 @@ class HairBatch:
 
     def set_launch_shape(self, vertices_per_strand, variant, blocks_per_cu=0):
         _check(self._lib.tfx_batch_set_launch_shape(self.handle, vertices_per_strand, variant, blocks_per_cu))
+
+    def autotune(self, stream=None, repeats=2):
+        """Times every launch shape of each strand length on the staged frame's
+        first substep and pins the fastest (results are unaffected); call
+        between stage_substep and run_frame.  Returns one dict per strand length."""
+        recs = (AutotuneRecord * 8)()
+        n = ctypes.c_int()
+        _check(self._lib.tfx_batch_autotune(self.handle, stream, int(repeats), recs, 8, ctypes.byref(n)))
+        return [{f: getattr(recs[i], f) for f, _ in AutotuneRecord._fields_} for i in range(min(n.value, 8))]
 
     def begin_frame(self):
         _check(self._lib.tfx_batch_begin_frame(self.handle))

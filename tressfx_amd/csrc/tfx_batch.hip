@@ -79,6 +79,53 @@ static void release_host(tfx_batch* b) {
     b->host_bytes = 0;
 }
 
+// Work items (member, first strand) of the strand blocks of every member of
+// strand length V, for blocks of B strands: large objects first, so their
+// blocks start early and the tail is made of the small ones.
+static void class_items(const tfx_batch* b, int V, int B, std::vector<tfx::WorkItem>& items) {
+    const int n = (int)b->members.size();
+    std::vector<int> order;
+    for (int m = 0; m < n; ++m)
+        if (b->members[m].hair->V == V) order.push_back(m);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return b->members[x].hair->G > b->members[y].hair->G; });
+    for (int m : order)
+        for (int g0 = 0; g0 < b->members[m].hair->G; g0 += B) items.push_back(tfx::WorkItem{m, g0});
+}
+
+// Work lists per strand-length class (identical for every substep), for the
+// launch shapes the batch's policy gives.
+static int build_work_lists(tfx_batch* b) {
+    const int n = (int)b->members.size();
+    std::vector<tfx_hair*> hairs(n);
+    for (int m = 0; m < n; ++m) hairs[m] = b->members[m].hair;
+    std::vector<tfx::WorkItem> items;
+    const int cus = cu_count(b->device);
+    for (int V = 4; V <= 64; V *= 2) {
+        b->vbegin[V] = (int)items.size();
+        b->vcount[V] = 0;
+        long long guides = 0;
+        for (int m = 0; m < n; ++m)
+            if (hairs[m]->V == V) guides += hairs[m]->G;
+        if (!guides) continue;
+        const Shape sh = choose_shape(&b->shape, V, hairs.data(), n, cus);
+        b->vvariant[V] = sh.variant;
+        b->vresident[V] = sh.resident;
+        class_items(b, V, kl_fused_block_vertices(V, sh.variant) / V, items);
+        b->vcount[V] = (int)items.size() - b->vbegin[V];
+    }
+    if (b->dev_items) {
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(b->dev_items));
+        b->dev_items = nullptr;
+    }
+    if (!items.empty()) {
+        HIP_TRY(hipMalloc((void**)&b->dev_items, items.size() * sizeof(tfx::WorkItem)));
+        HIP_TRY(hipMemcpy(b->dev_items, items.data(), items.size() * sizeof(tfx::WorkItem), hipMemcpyHostToDevice));
+    }
+    return TFX_OK;
+}
+
 // Lays out slots, tables and work lists for the current membership.
 static int prepare(tfx_batch* b) {
     if (!b->dirty) return TFX_OK;
@@ -131,36 +178,8 @@ static int prepare(tfx_batch* b) {
         HIP_TRY(hipMalloc((void**)&b->dev_tables, tables.size() * sizeof(DevHair)));
         HIP_TRY(hipMemcpy(b->dev_tables, tables.data(), tables.size() * sizeof(DevHair), hipMemcpyHostToDevice));
     }
-    // Work lists per strand-length class (identical for every substep).
-    std::vector<tfx_hair*> hairs(n);
-    for (int m = 0; m < n; ++m) hairs[m] = b->members[m].hair;
-    std::vector<tfx::WorkItem> items;
-    const int cus = cu_count(b->device);
-    for (int V = 4; V <= 64; V *= 2) {
-        b->vbegin[V] = (int)items.size();
-        b->vcount[V] = 0;
-        long long guides = 0;
-        for (int m = 0; m < n; ++m)
-            if (hairs[m]->V == V) guides += hairs[m]->G;
-        if (!guides) continue;
-        const Shape sh = choose_shape(&b->shape, V, hairs.data(), n, cus);
-        b->vvariant[V] = sh.variant;
-        b->vresident[V] = sh.resident;
-        const int B = kl_fused_block_vertices(V, sh.variant) / V;
-        // Large objects first: their blocks start early and the tail is made
-        // of the small ones.
-        std::vector<int> order;
-        for (int m = 0; m < n; ++m)
-            if (hairs[m]->V == V) order.push_back(m);
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return hairs[x]->G > hairs[y]->G; });
-        for (int m : order)
-            for (int g0 = 0; g0 < hairs[m]->G; g0 += B) items.push_back(tfx::WorkItem{m, g0});
-        b->vcount[V] = (int)items.size() - b->vbegin[V];
-    }
-    if (!items.empty()) {
-        HIP_TRY(hipMalloc((void**)&b->dev_items, items.size() * sizeof(tfx::WorkItem)));
-        HIP_TRY(hipMemcpy(b->dev_items, items.data(), items.size() * sizeof(tfx::WorkItem), hipMemcpyHostToDevice));
-    }
+    int rc = build_work_lists(b);
+    if (rc) return rc;
     b->dirty = false;
     return TFX_OK;
 }
@@ -170,7 +189,7 @@ extern "C" int tfx_batch_create(tfx_batch** out) {
     tfx_batch* b = new tfx_batch();
     b->device = g_device;
     b->shape.device = g_device;
-    for (int v = 0; v <= 64; ++v) b->shape.variant[v] = -1;
+    for (int v = 0; v <= 64; ++v) b->shape.variant[v] = b->shape.per_cu[v] = -1;
     *out = b;
     return TFX_OK;
 }
@@ -316,6 +335,132 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
                                      b->vcount[V], grid, s));
         }
     return TFX_OK;
+}
+
+// Autotuner of the launch shapes: per strand-length class, every distinct
+// strand-block size and every persistent-blocks-per-CU count up to the
+// kernel's occupancy runs the class's fused launch of the staged frame's first
+// substep `repeats` times (HIP events on the stream, fastest run kept), each
+// from the members' state as it was before the call (a snapshot, restored
+// before every run and at the end), and the fastest shape is pinned for the
+// class.
+extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_autotune_record* out, int max_out,
+                                  int* num_out) {
+    CHECK_ARG(b && num_out && (out || max_out == 0), "tfx_batch_autotune: null argument");
+    CHECK_ARG(b->slot >= 0 && b->staged > 0, "tfx_batch_autotune: stage a frame first (begin_frame, stage_substep)");
+    CHECK_ARG(repeats >= 1 && repeats <= 16, "tfx_batch_autotune: repeats must be 1..16");
+    *num_out = 0;
+    const int n = (int)b->members.size();
+    if (n == 0) return TFX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    // the staged parameters of substep 0 on the device (run_frame uploads them again)
+    HIP_TRY(hipMemcpyAsync(b->dev_slot, b->host[b->slot], b->off_sub + b->sub_bytes, hipMemcpyHostToDevice, s));
+    // snapshot of the state the fused kernel writes
+    struct Buf { float4* p; size_t bytes; size_t at; int V; };
+    std::vector<Buf> bufs;
+    size_t total = 0;
+    for (int m = 0; m < n; ++m) {
+        tfx_hair* h = b->members[m].hair;
+        for (float4* p : {h->pos, h->prev, h->prevprev, h->tangent}) {
+            bufs.push_back(Buf{p, (size_t)h->N * sizeof(float4), total, h->V});
+            total += (size_t)h->N * sizeof(float4);
+        }
+    }
+    char* snap = nullptr;
+    tfx::WorkItem* items_d = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TFX_OK;
+    auto cleanup = [&]() {
+        if (snap) {
+            for (const Buf& x : bufs) (void)hipMemcpyAsync(x.p, snap + x.at, x.bytes, hipMemcpyDeviceToDevice, s);
+            (void)hipStreamSynchronize(s);
+        }
+        (void)hipFree(snap);
+        (void)hipFree(items_d);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    auto hip_fail = [&](hipError_t e, const char* what) {
+        cleanup();
+        return fail(e == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_batch_autotune: %s: %s",
+                    what, hipGetErrorString(e));
+    };
+    hipError_t e = hipMalloc((void**)&snap, std::max<size_t>(total, 1));
+    if (e != hipSuccess) { snap = nullptr; return hip_fail(e, "snapshot"); }
+    for (const Buf& x : bufs)
+        if ((e = hipMemcpyAsync(snap + x.at, x.p, x.bytes, hipMemcpyDeviceToDevice, s)) != hipSuccess)
+            return hip_fail(e, "snapshot");
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+        return hip_fail(e, "events");
+    std::vector<tfx_hair*> hairs(n);
+    for (int m = 0; m < n; ++m) hairs[m] = b->members[m].hair;
+    const int cus = cu_count(b->device);
+    size_t items_cap = 0;
+    for (int V = 4; V <= 64; V *= 2) {
+        bool any = false;
+        for (int m = 0; m < n; ++m) any = any || hairs[m]->V == V;
+        if (!any) continue;
+        const Shape def = choose_shape(&b->shape, V, hairs.data(), n, cus);
+        const int def_per_cu = std::max(1, def.resident / std::max(cus, 1));
+        tfx_autotune_record rec{V, def.variant, def_per_cu, 0, 0.0f, 0.0f};
+        float best = -1.0f;
+        for (int variant = 0; variant <= 2; ++variant) {
+            const int vb = kl_fused_block_vertices(V, variant);
+            if (variant > 0 && vb == kl_fused_block_vertices(V, variant - 1)) continue;
+            std::vector<tfx::WorkItem> items;
+            class_items(b, V, vb / V, items);
+            if (items.size() > items_cap) {
+                (void)hipFree(items_d);
+                items_d = nullptr;
+                if ((e = hipMalloc((void**)&items_d, items.size() * sizeof(tfx::WorkItem))) != hipSuccess)
+                    return hip_fail(e, "work list");
+                items_cap = items.size();
+            }
+            if ((e = hipMemcpy(items_d, items.data(), items.size() * sizeof(tfx::WorkItem), hipMemcpyHostToDevice)) !=
+                hipSuccess)
+                return hip_fail(e, "work list");
+            const int occ = std::max(1, kl_fused_occupancy(V, variant));
+            for (int per_cu = 1; per_cu <= occ; ++per_cu) {
+                const int grid = std::min((int)items.size(), per_cu * cus);
+                float t = -1.0f;
+                for (int r = 0; r < repeats; ++r) {
+                    for (const Buf& x : bufs)
+                        if (x.V == V && (e = hipMemcpyAsync(x.p, snap + x.at, x.bytes, hipMemcpyDeviceToDevice, s)) !=
+                                            hipSuccess)
+                            return hip_fail(e, "restore");
+                    if ((e = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(e, "event");
+                    if ((e = kl_launch_fused(V, variant, b->dev_tables, items_d, (int)items.size(), grid, s)) !=
+                        hipSuccess)
+                        return hip_fail(e, "launch");
+                    if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
+                        return hip_fail(e, "event");
+                    float ms = 0.0f;
+                    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) return hip_fail(e, "event");
+                    if (t < 0.0f || ms < t) t = ms;
+                }
+                rec.candidates += 1;
+                if (variant == def.variant && per_cu == def_per_cu) rec.default_us = 1e3f * t;
+                if (best < 0.0f || t < best) {
+                    best = t;
+                    rec.variant = variant;
+                    rec.blocks_per_cu = per_cu;
+                }
+            }
+        }
+        rec.chosen_us = 1e3f * best;
+        b->shape.variant[V] = rec.variant;
+        b->shape.per_cu[V] = rec.blocks_per_cu;
+        if (*num_out < max_out) out[*num_out] = rec;
+        *num_out += 1;
+    }
+    cleanup();
+    // the work lists and captured frames follow the pinned shapes
+    for (int k = 0; k < 2; ++k) {
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        b->exec[k] = nullptr;
+    }
+    rc = build_work_lists(b);
+    return rc;
 }
 
 // Runs the staged frame: parameter upload, batched SDF build, and the staged

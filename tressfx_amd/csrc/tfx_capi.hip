@@ -397,7 +397,7 @@ extern "C" int tfx_simulation_create(tfx_simulation** out) {
     CHECK_ARG(out, "tfx_simulation_create: null argument");
     tfx_simulation* s = new tfx_simulation();
     s->device = g_device;
-    for (int v = 0; v <= 64; ++v) s->variant[v] = -1;
+    for (int v = 0; v <= 64; ++v) s->variant[v] = s->per_cu[v] = -1;
     *out = s;
     return TFX_OK;
 }
@@ -450,6 +450,7 @@ extern "C" int tfx_simulation_set_launch_shape(tfx_simulation* sim, int V, int v
     CHECK_ARG(blocks_per_cu >= 0 && blocks_per_cu <= 64, "tfx_simulation_set_launch_shape: blocks_per_cu out of range");
     sim->variant[V] = variant;
     sim->grid_per_cu = blocks_per_cu;
+    sim->per_cu[V] = -1;
     return TFX_OK;
 }
 

@@ -225,6 +225,7 @@ struct tfx_simulation {
     size_t table_bytes = 0;
     int variant[65];           // launch-shape variant per V (-1: chosen per call from occupancy)
     int grid_per_cu = 0;       // persistent blocks per CU (0: the kernel's occupancy)
+    int per_cu[65];            // per V, overrides grid_per_cu when >= 0 (set by the batch autotuner)
 };
 
 struct tfx_sdf_collision_system {
@@ -462,7 +463,8 @@ struct Shape { int variant; int resident; };
 inline Shape choose_shape(const tfx_simulation* sim, int V, tfx_hair* const* hairs, int n, int cus) {
     auto resident_of = [&](int variant) {
         const int occ = kl_fused_occupancy(V, variant);
-        const int per_cu = sim->grid_per_cu > 0 ? std::min(sim->grid_per_cu, std::max(occ, 1)) : std::max(occ, 1);
+        const int cap = sim->per_cu[V] >= 0 ? sim->per_cu[V] : sim->grid_per_cu;
+        const int per_cu = cap > 0 ? std::min(cap, std::max(occ, 1)) : std::max(occ, 1);
         return per_cu * cus;
     };
     if (sim->variant[V] >= 0) return Shape{sim->variant[V], resident_of(sim->variant[V])};

@@ -489,7 +489,7 @@ extern "C" int tfx_server_create(const tfx_server_config* cfg, tfx_server** out)
     s->cfg.max_inflight = std::min(std::max(cfg->max_inflight, 1), 4);
     HIP_TRY(hipSetDevice(cfg->device));
     s->shape.device = cfg->device;
-    for (int v = 0; v <= 64; ++v) s->shape.variant[v] = -1;
+    for (int v = 0; v <= 64; ++v) s->shape.variant[v] = s->shape.per_cu[v] = -1;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete s;

@@ -152,7 +152,9 @@ class BatchDriver:
     def vertices(self):
         return sum(wl.asset.num_total_vertices for wl in self.wls)
 
-    def frame(self, f, stream=None, use_graph=False):
+    def frame(self, f, stream=None, use_graph=False, autotune_repeats=0):
+        """One frame; with autotune_repeats > 0 the batch autotuner runs on it
+        first (records in self.autotune_records; the results are unaffected)."""
         k = f - self.first
         for sdf, d in zip(self.sdfs, self.descs[k]):
             if sdf is not None:
@@ -161,6 +163,8 @@ class BatchDriver:
         for s in range(self.substeps):
             bones, st = self.inputs[(f, s)]
             self.batch.stage_substep(bones, st, self.dt_sub)
+        if autotune_repeats > 0:
+            self.autotune_records = self.batch.autotune(stream, repeats=autotune_repeats)
         self.batch.run_frame(stream, use_graph)
 
 
