@@ -5,6 +5,12 @@ FETCH_SIZE and WRITE_SIZE, calibrated against a device copy of known size.
     python -m tressfx_amd.profile_summary stats  KERNEL_STATS_CSV KERNEL_SUBSTRING
     python -m tressfx_amd.profile_summary traffic FETCH_CSV WRITE_CSV KERNEL_SUBSTRING COPY_BYTES OUT_JSON
     python -m tressfx_amd.profile_summary digest
+    python -m tressfx_amd.profile_summary counters COUNTER_CSV KERNEL_SUBSTRING
+
+`counters` prints, for the dispatches of the kernel, the mean per dispatch of
+every counter of a --pmc run and the ratios read from SQ counters: VALU and
+LDS instructions per wave, VALU and LDS issue per wave-cycle, LDS bank
+conflicts per LDS issue cycle.
 
 Counter values are KiB per dispatch.  The copy kernel of the same run
 (__amd_rocclr_copyBuffer, COPY_BYTES read and written) gives the scale factor
@@ -86,6 +92,30 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
     return res
 
 
+def counters(path, kernel):
+    names = sorted({r["Counter_Name"] for r in _rows(path)})
+    out = {}
+    for c in names:
+        ks = [v for n, vs in _per_dispatch(path, c).items() if kernel in n for v in vs]
+        if ks:
+            out[c] = {"mean_per_dispatch": statistics.mean(ks), "dispatches": len(ks)}
+    m = {c: v["mean_per_dispatch"] for c, v in out.items()}
+    ratios = {}
+    if m.get("SQ_WAVES"):
+        for c, label in (("SQ_INSTS_VALU", "valu_insts_per_wave"), ("SQ_INSTS_LDS", "lds_insts_per_wave")):
+            if c in m:
+                ratios[label] = m[c] / m["SQ_WAVES"]
+    if m.get("SQ_WAVE_CYCLES"):
+        for c, label in (("SQ_ACTIVE_INST_VALU", "valu_issue_per_wave_cycle"),
+                         ("SQ_ACTIVE_INST_LDS", "lds_issue_per_wave_cycle"),
+                         ("SQ_WAIT_INST_LDS", "lds_wait_per_wave_cycle")):
+            if c in m:
+                ratios[label] = m[c] / m["SQ_WAVE_CYCLES"]
+    if m.get("SQ_ACTIVE_INST_LDS") and "SQ_LDS_BANK_CONFLICT" in m:
+        ratios["lds_bank_conflict_per_lds_issue_cycle"] = m["SQ_LDS_BANK_CONFLICT"] / m["SQ_ACTIVE_INST_LDS"]
+    return {"kernel": kernel, "counters": out, "ratios": ratios}
+
+
 def main(argv):
     if argv[:1] == ["stats"]:
         for k in kernel_stats(argv[1], argv[2]):
@@ -95,6 +125,8 @@ def main(argv):
         with open(argv[5], "w") as f:
             json.dump(r, f, indent=1)
         print(json.dumps(r))
+    elif argv[:1] == ["counters"]:
+        print(json.dumps(counters(argv[1], argv[2]), indent=1))
     elif argv[:1] == ["digest"]:
         print(fused_source_digest())
     else:
