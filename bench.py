@@ -442,7 +442,6 @@ def main():
             brun = harness.ServedRun(srv, wls3, initial=0, duration_s=0.0, inst_rate=0.0, req_rate=0.0)
             _log(f"served capacity: {len(wls3)} instances x {args.served_burst_frames} frames queued at once")
             bres = brun.burst(args.served_burst_frames)
-            srv.close()
             bv = bres["vertex_substeps"] / bres["window_s"]
             served["capacity"] = {
                 "workload": f"the {len(wls3)} config-3 instances resident, {args.served_burst_frames} frames of each "
@@ -452,6 +451,18 @@ def main():
                 "mean_instances_per_pass": bres["mean_instances_per_pass"],
                 "speedup_vs_plain": (bv / batched["plain"]["value"]) if batched and batched.get("plain") else None,
             }
+            if not args.no_fast:
+                tfx.set_arithmetic(tfx.TFX_ARITH_FAST)
+                try:
+                    fres = brun.burst(args.served_burst_frames)
+                finally:
+                    tfx.set_arithmetic(tfx.TFX_ARITH_IEEE)
+                fv = fres["vertex_substeps"] / fres["window_s"]
+                served["capacity"]["fast_arithmetic"] = {
+                    "value": fv, "window_s": fres["window_s"], "passes": fres["passes"],
+                    "speedup_vs_plain": (fv / batched["plain"]["value"]) if batched and batched.get("plain") else None,
+                }
+            srv.close()
 
     fur = None
     if wl5 is not None:
