@@ -483,6 +483,16 @@ int tfx_batch_remove(tfx_batch* batch, tfx_hair* hair);
 int tfx_batch_count(const tfx_batch* batch, int* num_members);
 /* As tfx_simulation_set_launch_shape, for the batch's fused launches. */
 int tfx_batch_set_launch_shape(tfx_batch* batch, int vertices_per_strand, int variant, int blocks_per_cu);
+/* Where the batch runs the SDF collision (CollideHairVerticesWithSdf_forward,
+ * TressFXSDFCollision.hlsl:531-595) of a strand-length class: inside its fused
+ * launch, or in a collide pass launched after each substep's fused launches
+ * (the reference's order: the substep's Simulate, then CollideWithHair).
+ * Results are identical either way.  TFX_SDF_DEFER_AUTO (default) defers for
+ * a class whose strand blocks fit in one round of resident blocks. */
+#define TFX_SDF_DEFER_AUTO (-1)
+#define TFX_SDF_DEFER_NEVER 0
+#define TFX_SDF_DEFER_ALWAYS 1
+int tfx_batch_set_sdf_deferral(tfx_batch* batch, int mode);
 /* Autotuner of the batch's launch shapes (no counterpart in the reference).
  * Call with a frame staged (after tfx_batch_stage_substep, before
  * tfx_batch_run_frame), best after frames have run so that states and SDFs are

@@ -114,6 +114,34 @@ def test_batch_direct_and_graph_equal_per_object_dispatch(gpu):
          stream_of=lambda c: None if c == 1 else side.cuda_stream, graph_of=lambda c: c == 2)
 
 
+def test_batch_sdf_deferral_modes_equal_per_object_dispatch(gpu):
+    """The SDF collision inside the fused launches (TFX_SDF_DEFER_NEVER) and in
+    a collide pass after each substep's fused launches (_ALWAYS; _AUTO picks
+    it for these small classes, which fit in one round of resident blocks)
+    give the per-object dispatch's results bit for bit, directly and from a
+    graph, the deferral changed between frames of one batch included."""
+    torch = gpu
+    print("[batch] SDF deferral never/always/auto vs per-object, 6 members, 4 frames", flush=True)
+    members = [Member(torch, i, *spec, copies=4) for i, spec in enumerate(MEMBERS)]
+    side = torch.cuda.Stream()
+    modes = {1: tfx.TFX_SDF_DEFER_NEVER, 2: tfx.TFX_SDF_DEFER_ALWAYS, 3: tfx.TFX_SDF_DEFER_AUTO}
+    batches = {}
+    for c, mode in modes.items():
+        b = tfx.HairBatch()
+        for m in members:
+            b.add(m.hairs[c], m.sdfs[c])
+        b.set_sdf_deferral(mode)
+        batches[c] = b
+    _run(torch, members, range(2), lambda f: 2, plain_copy=0, batch_copies=tuple(modes), batches=batches,
+         stream_of=lambda c: side.cuda_stream, graph_of=lambda c: c != 1)
+    batches[1].set_sdf_deferral(tfx.TFX_SDF_DEFER_ALWAYS)
+    batches[2].set_sdf_deferral(tfx.TFX_SDF_DEFER_NEVER)
+    _run(torch, members, range(2, 4), lambda f: 2, plain_copy=0, batch_copies=tuple(modes), batches=batches,
+         stream_of=lambda c: side.cuda_stream, graph_of=lambda c: c != 1)
+    with pytest.raises(tfx.TFXError):
+        batches[1].set_sdf_deferral(2)
+
+
 def test_batch_graph_recaptures_on_substep_change(gpu):
     """A frame with another substep count re-captures its slot's graph; both
     slots alternate between frames."""

@@ -289,6 +289,9 @@ TFX_SDF_COLLIDE_FORWARD = 0
 TFX_SDF_COLLIDE_MIXED = 1
 TFX_ARITH_IEEE = 0
 TFX_ARITH_FAST = 1
+TFX_SDF_DEFER_AUTO = -1
+TFX_SDF_DEFER_NEVER = 0
+TFX_SDF_DEFER_ALWAYS = 1
 
 EXPORTS = {
     "tfx_last_error": (ctypes.c_char_p, []),
@@ -372,6 +375,7 @@ EXPORTS = {
     "tfx_batch_remove": (_I, [_VP, _VP]),
     "tfx_batch_count": (_I, [_VP, ctypes.POINTER(_I)]),
     "tfx_batch_set_launch_shape": (_I, [_VP, _I, _I, _I]),
+    "tfx_batch_set_sdf_deferral": (_I, [_VP, _I]),
     "tfx_batch_autotune": (_I, [_VP, _VP, _I, _VP, _I, _VP]),
     "tfx_batch_begin_frame": (_I, [_VP]),
     "tfx_batch_stage_substep": (_I, [_VP, c_float_p, _I, ctypes.POINTER(SimulationSettings), _I, _F, c_float_p]),
@@ -817,6 +821,11 @@ class HairBatch:
 
     def set_launch_shape(self, vertices_per_strand, variant, blocks_per_cu=0):
         _check(self._lib.tfx_batch_set_launch_shape(self.handle, vertices_per_strand, variant, blocks_per_cu))
+
+    def set_sdf_deferral(self, mode):
+        """TFX_SDF_DEFER_AUTO, _NEVER or _ALWAYS: SDF collision inside the
+        fused launches or in a collide pass after them (same results)."""
+        _check(self._lib.tfx_batch_set_sdf_deferral(self.handle, int(mode)))
 
     def autotune(self, stream=None, repeats=2):
         """Times every launch shape of each strand length on the staged frame's

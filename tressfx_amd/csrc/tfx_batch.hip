@@ -42,6 +42,8 @@ struct tfx_batch {
     DevHair* dev_tables = nullptr;     // [cap][n]
     tfx::WorkItem* dev_items = nullptr;
     int vbegin[65] = {0}, vcount[65] = {0}, vvariant[65] = {0}, vresident[65] = {0};
+    int vdefer[65] = {0};              // class V's fused launch leaves the SDF collision to the collide pass
+    int cbegin = 0, ccount = 0;        // collide pass work items (member, first vertex) in dev_items
 
     void* host[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -114,6 +116,25 @@ static int build_work_lists(tfx_batch* b) {
         class_items(b, V, kl_fused_block_vertices(V, sh.variant) / V, items);
         b->vcount[V] = (int)items.size() - b->vbegin[V];
     }
+    // SDF collision deferred to a collide pass after each substep's fused
+    // launches, per class: by default where the class's strand blocks fit in
+    // one round of resident blocks, so that the fused launch's duration is one
+    // block's latency and the lookups, taken inside a strand block one group
+    // after another, would add to it in full.
+    b->cbegin = (int)items.size();
+    const int chunk = kl_collide_chunk();
+    for (int V = 4; V <= 64; V *= 2) {
+        const int mode = b->shape.defer_sdf;
+        b->vdefer[V] = b->vcount[V] > 0 && (mode == 1 || (mode < 0 && b->vcount[V] <= b->vresident[V]));
+        if (!b->vdefer[V]) continue;
+        for (int m = 0; m < n; ++m) {
+            const tfx_hair* h = hairs[m];
+            if (h->V != V || !b->members[m].sdf) continue;
+            const long long nv = (long long)h->G * (h->F + 1) * V;
+            for (long long v0 = 0; v0 < nv; v0 += chunk) items.push_back(tfx::WorkItem{m, (int)v0});
+        }
+    }
+    b->ccount = (int)items.size() - b->cbegin;
     if (b->dev_items) {
         HIP_TRY(hipDeviceSynchronize());
         HIP_TRY(hipFree(b->dev_items));
@@ -231,6 +252,15 @@ extern "C" int tfx_batch_count(const tfx_batch* b, int* n) {
     return TFX_OK;
 }
 
+extern "C" int tfx_batch_set_sdf_deferral(tfx_batch* b, int mode) {
+    CHECK_ARG(b, "tfx_batch_set_sdf_deferral: null batch");
+    CHECK_ARG(mode >= TFX_SDF_DEFER_AUTO && mode <= TFX_SDF_DEFER_ALWAYS,
+              "tfx_batch_set_sdf_deferral: mode must be TFX_SDF_DEFER_AUTO, _NEVER or _ALWAYS");
+    b->shape.defer_sdf = mode;
+    b->dirty = true;
+    return TFX_OK;
+}
+
 extern "C" int tfx_batch_set_launch_shape(tfx_batch* b, int V, int variant, int blocks_per_cu) {
     CHECK_ARG(b, "tfx_batch_set_launch_shape: null batch");
     int rc = tfx_simulation_set_launch_shape(&b->shape, V, variant, blocks_per_cu);
@@ -326,14 +356,20 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
             HIP_TRY(kl_launch_sdf_multi(phase, jobs, nsdf, b->max_cells, b->max_tris, s));
         }
     }
-    for (int sub = 0; sub < substeps; ++sub)
+    for (int sub = 0; sub < substeps; ++sub) {
+        const DevHair* tables = b->dev_tables + (size_t)sub * n;
         for (int V = 4; V <= 64; V *= 2) {
             if (!b->vcount[V]) continue;
             const int grid = std::min(b->vcount[V], b->vresident[V]);
             PassTimer tm(timers ? "FusedSubstep" : nullptr, s);
-            HIP_TRY(kl_launch_fused(V, b->vvariant[V], b->dev_tables + (size_t)sub * n, b->dev_items + b->vbegin[V],
-                                     b->vcount[V], grid, s));
+            HIP_TRY(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V], b->vcount[V], grid,
+                                     b->vdefer[V], s));
         }
+        if (b->ccount) {
+            PassTimer tm(timers ? "CollideHairVerticesWithSdf" : nullptr, s);
+            HIP_TRY(kl_launch_collide_items(tables, b->dev_items + b->cbegin, b->ccount, s));
+        }
+    }
     return TFX_OK;
 }
 
@@ -429,7 +465,7 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
                                             hipSuccess)
                             return hip_fail(e, "restore");
                     if ((e = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(e, "event");
-                    if ((e = kl_launch_fused(V, variant, b->dev_tables, items_d, (int)items.size(), grid, s)) !=
+                    if ((e = kl_launch_fused(V, variant, b->dev_tables, items_d, (int)items.size(), grid, 0, s)) !=
                         hipSuccess)
                         return hip_fail(e, "launch");
                     if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)

@@ -39,7 +39,8 @@ struct FusedShape {
 
 template <int V, int VB>
 __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair* __restrict__ hairs,
-                                                               const WorkItem* __restrict__ items, int nitems) {
+                                                               const WorkItem* __restrict__ items, int nitems,
+                                                               int defer_sdf) {
     using S = FusedShape<V, VB>;
     constexpr int B = S::B, NPT = S::NPT, STRIDE = S::STRIDE;
     __shared__ float4 sPos[B * STRIDE];
@@ -71,7 +72,11 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
         const int nb = min(B, h.G - g0);
 
         // SDF brick occupancy into LDS (read by P6's conservative early-out).
-        const bool has_sdf = P.has_sdf != 0;
+        // defer_sdf: the SDF collision is left to k_sdf_collide_items, run
+        // after the substep (the reference's own order: Simulate, then
+        // CollideWithHair; the tangents and follow strands come from the
+        // positions before the collision either way).
+        const bool has_sdf = P.has_sdf != 0 && !defer_sdf;
         const int brick_words = (P.sdf_bx * P.sdf_by * P.sdf_bz + 31) >> 5;
         const bool bits_lds = has_sdf && h.sdf_brick_bits != nullptr && brick_words <= kMaxBrickWords;
         if (bits_lds)
@@ -520,10 +525,45 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
 }
 
 template <int V, int VB>
-static hipError_t launch_fused(const DevHair* hairs, const WorkItem* items, int nitems, int grid, hipStream_t stream) {
+static hipError_t launch_fused(const DevHair* hairs, const WorkItem* items, int nitems, int grid, int defer_sdf,
+                               hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_fused_substep<V, VB>), dim3(grid), dim3(kFusedBlock), 0, stream, hairs, items, nitems);
+    hipLaunchKernelGGL((k_fused_substep<V, VB>), dim3(grid), dim3(kFusedBlock), 0, stream, hairs, items, nitems,
+                       defer_sdf);
     return hipGetLastError();
+}
+
+// CollideHairVerticesWithSdf_forward :531-595 for the objects whose fused
+// launch deferred it: one 64-lane block per work item (hair, first vertex),
+// one vertex per lane, local vertices 0 and 1 skipped.  A lane's lookups form
+// one dependent chain (brick occupancy, then the first corner, then the other
+// seven, then the gradient); many small blocks keep many chains in flight,
+// where the fused launch takes them one group after another inside each
+// strand block.
+constexpr int kCollideBlock = 64;
+
+__global__ void __launch_bounds__(kCollideBlock) k_sdf_collide_items(const DevHair* __restrict__ hairs,
+                                                                     const WorkItem* __restrict__ items) {
+    const WorkItem item = items[blockIdx.x];
+    const int hi = __builtin_amdgcn_readfirstlane(item.hair);
+    const int v0 = __builtin_amdgcn_readfirstlane(item.g0);
+    const DevHair h = load_uniform(hairs + hi);
+    const_as<DevParams>& P = *uniform_const(h.params);
+    const int i = v0 + (int)threadIdx.x;
+    const int V = h.V, n = h.G * (h.F + 1) * V;
+    if (P.has_sdf == 0 || i >= n || i % V < 2) return;
+    SdfGrid grid;
+    grid.ox = P.sdf_ox; grid.oy = P.sdf_oy; grid.oz = P.sdf_oz; grid.cell = P.sdf_cell;
+    grid.nx = P.sdf_nx; grid.ny = P.sdf_ny; grid.nz = P.sdf_nz; grid.pad = 0;
+    grid.bricks = h.sdf_bricks;
+    grid.brick_bits = nullptr;
+    grid.inv_cell = P.sdf_inv_cell;
+    grid.bx = P.sdf_bx; grid.by = P.sdf_by; grid.bz = P.sdf_bz;
+    f3 out;
+    if (sdf_collide_vertex<false>(grid, h.sdf, P.sdf_margin, xyz(ldg(&h.pos[i])), out)) {
+        store_xyz(&h.pos[i], out);
+        store_xyz(&h.prev[i], out);
+    }
 }
 
 }  // namespace TFX_KERNEL_NS
@@ -538,23 +578,34 @@ extern "C" int TFX_KSYM(tfx_fused_block_vertices)(int V, int variant) {
     return vb / V > 128 ? 128 * V : vb;
 }
 
+extern "C" hipError_t TFX_KSYM(tfx_launch_collide_items)(const tfx::DevHair* hairs, const tfx::WorkItem* items,
+                                                         int nitems, hipStream_t stream) {
+    using namespace tfx;
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sdf_collide_items, dim3(nitems), dim3(kCollideBlock), 0, stream, hairs, items);
+    return hipGetLastError();
+}
+
+// Vertices per work item of tfx_launch_collide_items.
+extern "C" int TFX_KSYM(tfx_collide_chunk)() { return tfx::kCollideBlock; }
+
 extern "C" hipError_t TFX_KSYM(tfx_launch_fused)(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
-                                       int nitems, int grid, hipStream_t stream) {
+                                       int nitems, int grid, int defer_sdf, hipStream_t stream) {
     using namespace tfx;
     const int vb = TFX_KSYM(tfx_fused_block_vertices)(V, variant);
     switch (V * 10000 + vb) {
-        case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, stream);
-        case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, stream);
-        case 8 * 10000 + 1024: return launch_fused<8, 1024>(hairs, items, nitems, grid, stream);
-        case 16 * 10000 + 512: return launch_fused<16, 512>(hairs, items, nitems, grid, stream);
-        case 16 * 10000 + 1024: return launch_fused<16, 1024>(hairs, items, nitems, grid, stream);
-        case 16 * 10000 + 2048: return launch_fused<16, 2048>(hairs, items, nitems, grid, stream);
-        case 32 * 10000 + 512: return launch_fused<32, 512>(hairs, items, nitems, grid, stream);
-        case 32 * 10000 + 1024: return launch_fused<32, 1024>(hairs, items, nitems, grid, stream);
-        case 32 * 10000 + 2048: return launch_fused<32, 2048>(hairs, items, nitems, grid, stream);
-        case 64 * 10000 + 512: return launch_fused<64, 512>(hairs, items, nitems, grid, stream);
-        case 64 * 10000 + 1024: return launch_fused<64, 1024>(hairs, items, nitems, grid, stream);
-        case 64 * 10000 + 2048: return launch_fused<64, 2048>(hairs, items, nitems, grid, stream);
+        case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 8 * 10000 + 1024: return launch_fused<8, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 16 * 10000 + 512: return launch_fused<16, 512>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 16 * 10000 + 1024: return launch_fused<16, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 16 * 10000 + 2048: return launch_fused<16, 2048>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 32 * 10000 + 512: return launch_fused<32, 512>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 32 * 10000 + 1024: return launch_fused<32, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 32 * 10000 + 2048: return launch_fused<32, 2048>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 64 * 10000 + 512: return launch_fused<64, 512>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 64 * 10000 + 1024: return launch_fused<64, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 64 * 10000 + 2048: return launch_fused<64, 2048>(hairs, items, nitems, grid, defer_sdf, stream);
         default: return hipErrorInvalidValue;
     }
 }

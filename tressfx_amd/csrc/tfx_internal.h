@@ -53,8 +53,12 @@ TFX_DUAL(hipError_t, launch_sdf_collide_mixed,
          (pos, prev, n, sdf, g, margin, stream))
 TFX_DUAL(hipError_t, launch_fused,
          (int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, int grid,
-          hipStream_t stream),
-         (V, variant, hairs, items, nitems, grid, stream))
+          int defer_sdf, hipStream_t stream),
+         (V, variant, hairs, items, nitems, grid, defer_sdf, stream))
+TFX_DUAL(hipError_t, launch_collide_items,
+         (const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, hipStream_t stream),
+         (hairs, items, nitems, stream))
+TFX_DUAL(int, collide_chunk, (), ())
 TFX_DUAL(int, fused_block_vertices, (int V, int variant), (V, variant))
 TFX_DUAL(int, fused_occupancy, (int V, int variant), (V, variant))
 extern "C" hipError_t tfx_launch_sdf_multi(int phase, const tfx::SdfJob* jobs, int njobs, int max_cells, int max_tris,
@@ -226,6 +230,7 @@ struct tfx_simulation {
     int variant[65];           // launch-shape variant per V (-1: chosen per call from occupancy)
     int grid_per_cu = 0;       // persistent blocks per CU (0: the kernel's occupancy)
     int per_cu[65];            // per V, overrides grid_per_cu when >= 0 (set by the batch autotuner)
+    int defer_sdf = TFX_SDF_DEFER_AUTO;  // batched form: SDF collision in the fused launch or a pass after it
 };
 
 struct tfx_sdf_collision_system {

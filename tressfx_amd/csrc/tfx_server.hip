@@ -400,7 +400,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
             if (!cnt) continue;
             const int grid = std::min(cnt, shapes[V].resident);
             HIP_TRY(kl_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], cnt, grid,
-                                     st));
+                                     0, st));
         }
     HIP_TRY(hipEventRecord(pass.ev, st));
     if (!sl.ev) HIP_TRY(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
