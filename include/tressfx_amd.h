@@ -498,7 +498,8 @@ int tfx_batch_set_sdf_deferral(tfx_batch* batch, int mode);
  * tfx_batch_run_frame), best after frames have run so that states and SDFs are
  * representative.  For every strand-length class, each strand-block size and
  * each persistent-blocks-per-CU count up to the kernel's occupancy runs the
- * class's fused launch of the staged first substep `repeats` times from the
+ * class's fused launch of the staged first substep (and its collide pass,
+ * for a shape that defers the SDF collision) `repeats` times from the
  * members' state (restored after every run: results are unaffected), and the
  * fastest is pinned for the class until tfx_batch_set_launch_shape changes
  * it.  One record per class (at most max_out written, *num_out = classes). */
@@ -509,6 +510,7 @@ typedef struct tfx_autotune_record {
     int candidates;      /* shapes timed */
     float chosen_us;     /* the chosen shape's fastest launch */
     float default_us;    /* the shape in use before the call (0 if not among the candidates) */
+    int sdf_deferred;    /* the chosen shape defers the SDF collision (tfx_batch_set_sdf_deferral) */
 } tfx_autotune_record;
 int tfx_batch_autotune(tfx_batch* batch, void* stream, int repeats, tfx_autotune_record* out, int max_out,
                        int* num_out);

@@ -260,7 +260,8 @@ class AutotuneRecord(ctypes.Structure):
     """tfx_autotune_record: the launch shape the batch autotuner chose for one
     strand length, its fastest launch and that of the shape in use before."""
     _fields_ = [("vertices_per_strand", ctypes.c_int), ("variant", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("candidates", ctypes.c_int), ("chosen_us", ctypes.c_float), ("default_us", ctypes.c_float)]
+                ("candidates", ctypes.c_int), ("chosen_us", ctypes.c_float), ("default_us", ctypes.c_float),
+                ("sdf_deferred", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):

@@ -376,7 +376,8 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
 // Autotuner of the launch shapes: per strand-length class, every distinct
 // strand-block size and every persistent-blocks-per-CU count up to the
 // kernel's occupancy runs the class's fused launch of the staged frame's first
-// substep `repeats` times (HIP events on the stream, fastest run kept), each
+// substep (followed by the class's collide pass where that shape defers the
+// SDF collision) `repeats` times (HIP events on the stream, fastest run kept), each
 // from the members' state as it was before the call (a snapshot, restored
 // before every run and at the end), and the fastest shape is pinned for the
 // class.
@@ -404,6 +405,7 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
     }
     char* snap = nullptr;
     tfx::WorkItem* items_d = nullptr;
+    tfx::WorkItem* citems_d = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = TFX_OK;
     auto cleanup = [&]() {
@@ -413,6 +415,7 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
         }
         (void)hipFree(snap);
         (void)hipFree(items_d);
+        (void)hipFree(citems_d);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
     };
@@ -431,14 +434,31 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
     std::vector<tfx_hair*> hairs(n);
     for (int m = 0; m < n; ++m) hairs[m] = b->members[m].hair;
     const int cus = cu_count(b->device);
-    size_t items_cap = 0;
+    size_t items_cap = 0, citems_cap = 0;
+    const int chunk = kl_collide_chunk();
     for (int V = 4; V <= 64; V *= 2) {
         bool any = false;
         for (int m = 0; m < n; ++m) any = any || hairs[m]->V == V;
         if (!any) continue;
         const Shape def = choose_shape(&b->shape, V, hairs.data(), n, cus);
         const int def_per_cu = std::max(1, def.resident / std::max(cus, 1));
-        tfx_autotune_record rec{V, def.variant, def_per_cu, 0, 0.0f, 0.0f};
+        tfx_autotune_record rec{V, def.variant, def_per_cu, 0, 0.0f, 0.0f, 0};
+        // the class's collide pass, for the shapes that defer the SDF collision
+        std::vector<tfx::WorkItem> citems;
+        for (int m = 0; m < n; ++m)
+            if (hairs[m]->V == V && b->members[m].sdf)
+                for (long long v0 = 0; v0 < (long long)hairs[m]->N; v0 += chunk)
+                    citems.push_back(tfx::WorkItem{m, (int)v0});
+        if (citems.size() > citems_cap) {
+            (void)hipFree(citems_d);
+            citems_d = nullptr;
+            if ((e = hipMalloc((void**)&citems_d, citems.size() * sizeof(tfx::WorkItem))) != hipSuccess)
+                return hip_fail(e, "work list");
+            citems_cap = citems.size();
+        }
+        if (!citems.empty() && (e = hipMemcpy(citems_d, citems.data(), citems.size() * sizeof(tfx::WorkItem),
+                                              hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, "work list");
         float best = -1.0f;
         for (int variant = 0; variant <= 2; ++variant) {
             const int vb = kl_fused_block_vertices(V, variant);
@@ -458,6 +478,9 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
             const int occ = std::max(1, kl_fused_occupancy(V, variant));
             for (int per_cu = 1; per_cu <= occ; ++per_cu) {
                 const int grid = std::min((int)items.size(), per_cu * cus);
+                const int mode = b->shape.defer_sdf;
+                const int defer = !citems.empty() && (mode == TFX_SDF_DEFER_ALWAYS ||
+                                                      (mode == TFX_SDF_DEFER_AUTO && (int)items.size() <= per_cu * cus));
                 float t = -1.0f;
                 for (int r = 0; r < repeats; ++r) {
                     for (const Buf& x : bufs)
@@ -465,8 +488,10 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
                                             hipSuccess)
                             return hip_fail(e, "restore");
                     if ((e = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(e, "event");
-                    if ((e = kl_launch_fused(V, variant, b->dev_tables, items_d, (int)items.size(), grid, 0, s)) !=
-                        hipSuccess)
+                    if ((e = kl_launch_fused(V, variant, b->dev_tables, items_d, (int)items.size(), grid, defer,
+                                             s)) != hipSuccess ||
+                        (defer && (e = kl_launch_collide_items(b->dev_tables, citems_d, (int)citems.size(), s)) !=
+                                      hipSuccess))
                         return hip_fail(e, "launch");
                     if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
                         return hip_fail(e, "event");
@@ -480,6 +505,7 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
                     best = t;
                     rec.variant = variant;
                     rec.blocks_per_cu = per_cu;
+                    rec.sdf_deferred = defer;
                 }
             }
         }
