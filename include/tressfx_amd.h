@@ -605,7 +605,8 @@ int tfx_server_submit(tfx_server* server, int instance_id, int frames, const flo
  * latency_ms = completion - submission. */
 int tfx_server_poll(tfx_server* server, unsigned long long ticket, int* done, double* latency_ms);
 int tfx_server_wait(tfx_server* server, unsigned long long ticket, double* latency_ms);
-/* Blocks until nothing is queued or running. */
+/* Blocks until nothing is queued or running and the memory of retired
+ * instances is released. */
 int tfx_server_drain(tfx_server* server);
 int tfx_server_stats_read(tfx_server* server, tfx_server_stats* out, int reset);
 /* The instance's hair object and SDF (NULL if none), for inspection while it is idle. */

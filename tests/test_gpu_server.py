@@ -210,6 +210,26 @@ def test_admission_budget_and_retire(gpu):
     srv.close()
 
 
+def test_drain_waits_for_retired_instances_to_be_freed(gpu):
+    """Many instances retired in a row (the worker frees the first while the
+    next are retired) and drained: drain returns with every one freed and the
+    budget back at zero, and the server keeps serving."""
+    print("[server] retire many, drain", flush=True)
+    srv = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
+    made = [make_instance(20 + i, 60 + 10 * i, 8 if i % 2 else 16) for i in range(3)]
+    for rnd in range(2):
+        ids = [admit(srv, *made[i % 3]) for i in range(16)]
+        for i, iid in enumerate(ids):
+            srv.submit(iid, *inputs(made[i % 3][0], 0, 1), track=False)
+        srv.drain()
+        for iid in ids:
+            srv.retire(iid)
+        srv.drain()
+        st = srv.stats()
+        assert st["resident_instances"] == 0 and st["resident_bytes"] == 0, (rnd, st)
+    srv.close()
+
+
 def test_server_limits_and_errors(gpu):
     srv = tfx.HairServer(max_instances=1, substeps_per_frame=SUBSTEPS, substep_dt=float(DT))
     wl, mesh = make_instance(8, 100, 8)

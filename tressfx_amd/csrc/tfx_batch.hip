@@ -361,13 +361,13 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
         for (int V = 4; V <= 64; V *= 2) {
             if (!b->vcount[V]) continue;
             const int grid = std::min(b->vcount[V], b->vresident[V]);
-            PassTimer tm(timers ? "FusedSubstep" : nullptr, s);
+            PassTimer tm(timers ? "FusedSubstep" : nullptr, s, true);
             HIP_TRY(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V], b->vcount[V], grid,
-                                     b->vdefer[V], s));
+                                     b->vdefer[V], s, tm.start(), tm.stop()));
         }
         if (b->ccount) {
-            PassTimer tm(timers ? "CollideHairVerticesWithSdf" : nullptr, s);
-            HIP_TRY(kl_launch_collide_items(tables, b->dev_items + b->cbegin, b->ccount, s));
+            PassTimer tm(timers ? "CollideHairVerticesWithSdf" : nullptr, s, true);
+            HIP_TRY(kl_launch_collide_items(tables, b->dev_items + b->cbegin, b->ccount, s, tm.start(), tm.stop()));
         }
     }
     return TFX_OK;
@@ -489,9 +489,9 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
                             return hip_fail(e, "restore");
                     if ((e = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(e, "event");
                     if ((e = kl_launch_fused(V, variant, b->dev_tables, items_d, (int)items.size(), grid, defer,
-                                             s)) != hipSuccess ||
-                        (defer && (e = kl_launch_collide_items(b->dev_tables, citems_d, (int)citems.size(), s)) !=
-                                      hipSuccess))
+                                             s, nullptr, nullptr)) != hipSuccess ||
+                        (defer && (e = kl_launch_collide_items(b->dev_tables, citems_d, (int)citems.size(), s,
+                                                               nullptr, nullptr)) != hipSuccess))
                         return hip_fail(e, "launch");
                     if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
                         return hip_fail(e, "event");

@@ -517,8 +517,9 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         const int cnt = vclass_count[V];
         if (!cnt) continue;
         const int grid = std::min(cnt, vclass_resident[V]);
-        PassTimer tm("FusedSubstep", s);
-        HIP_TRY(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, 0, s));
+        PassTimer tm("FusedSubstep", s, true);
+        HIP_TRY(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, 0, s, tm.start(),
+                                 tm.stop()));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
     return TFX_OK;

@@ -19,6 +19,8 @@
 // RestPos are read once and Pos, Prev, Tangent written once per guide vertex;
 // follow strands are written once.  PrevPrev is written for vertices 0 and 1
 // only (the only ones any stage reads) and StrandLevelData stays on chip.
+#include <hip/hip_ext.h>
+
 #include "tfx_device.h"
 
 namespace tfx {
@@ -526,10 +528,14 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
 
 template <int V, int VB>
 static hipError_t launch_fused(const DevHair* hairs, const WorkItem* items, int nitems, int grid, int defer_sdf,
-                               hipStream_t stream) {
+                               hipStream_t stream, hipEvent_t t0, hipEvent_t t1) {
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_fused_substep<V, VB>), dim3(grid), dim3(kFusedBlock), 0, stream, hairs, items, nitems,
-                       defer_sdf);
+    if (t0)
+        hipExtLaunchKernelGGL((k_fused_substep<V, VB>), dim3(grid), dim3(kFusedBlock), 0, stream, t0, t1, 0, hairs,
+                              items, nitems, defer_sdf);
+    else
+        hipLaunchKernelGGL((k_fused_substep<V, VB>), dim3(grid), dim3(kFusedBlock), 0, stream, hairs, items, nitems,
+                           defer_sdf);
     return hipGetLastError();
 }
 
@@ -578,11 +584,17 @@ extern "C" int TFX_KSYM(tfx_fused_block_vertices)(int V, int variant) {
     return vb / V > 128 ? 128 * V : vb;
 }
 
+// t0, t1: events stamped by the launch itself (kernel begin and end), or null.
 extern "C" hipError_t TFX_KSYM(tfx_launch_collide_items)(const tfx::DevHair* hairs, const tfx::WorkItem* items,
-                                                         int nitems, hipStream_t stream) {
+                                                         int nitems, hipStream_t stream, hipEvent_t t0,
+                                                         hipEvent_t t1) {
     using namespace tfx;
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sdf_collide_items, dim3(nitems), dim3(kCollideBlock), 0, stream, hairs, items);
+    if (t0)
+        hipExtLaunchKernelGGL(k_sdf_collide_items, dim3(nitems), dim3(kCollideBlock), 0, stream, t0, t1, 0, hairs,
+                              items);
+    else
+        hipLaunchKernelGGL(k_sdf_collide_items, dim3(nitems), dim3(kCollideBlock), 0, stream, hairs, items);
     return hipGetLastError();
 }
 
@@ -590,22 +602,23 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_collide_items)(const tfx::DevHair* hai
 extern "C" int TFX_KSYM(tfx_collide_chunk)() { return tfx::kCollideBlock; }
 
 extern "C" hipError_t TFX_KSYM(tfx_launch_fused)(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
-                                       int nitems, int grid, int defer_sdf, hipStream_t stream) {
+                                       int nitems, int grid, int defer_sdf, hipStream_t stream,
+                                       hipEvent_t t0, hipEvent_t t1) {
     using namespace tfx;
     const int vb = TFX_KSYM(tfx_fused_block_vertices)(V, variant);
     switch (V * 10000 + vb) {
-        case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 8 * 10000 + 1024: return launch_fused<8, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 16 * 10000 + 512: return launch_fused<16, 512>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 16 * 10000 + 1024: return launch_fused<16, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 16 * 10000 + 2048: return launch_fused<16, 2048>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 32 * 10000 + 512: return launch_fused<32, 512>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 32 * 10000 + 1024: return launch_fused<32, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 32 * 10000 + 2048: return launch_fused<32, 2048>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 64 * 10000 + 512: return launch_fused<64, 512>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 64 * 10000 + 1024: return launch_fused<64, 1024>(hairs, items, nitems, grid, defer_sdf, stream);
-        case 64 * 10000 + 2048: return launch_fused<64, 2048>(hairs, items, nitems, grid, defer_sdf, stream);
+        case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 8 * 10000 + 1024: return launch_fused<8, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 16 * 10000 + 512: return launch_fused<16, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 16 * 10000 + 1024: return launch_fused<16, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 16 * 10000 + 2048: return launch_fused<16, 2048>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 32 * 10000 + 512: return launch_fused<32, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 32 * 10000 + 1024: return launch_fused<32, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 32 * 10000 + 2048: return launch_fused<32, 2048>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 64 * 10000 + 512: return launch_fused<64, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 64 * 10000 + 1024: return launch_fused<64, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 64 * 10000 + 2048: return launch_fused<64, 2048>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
         default: return hipErrorInvalidValue;
     }
 }

@@ -53,11 +53,12 @@ TFX_DUAL(hipError_t, launch_sdf_collide_mixed,
          (pos, prev, n, sdf, g, margin, stream))
 TFX_DUAL(hipError_t, launch_fused,
          (int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, int grid,
-          int defer_sdf, hipStream_t stream),
-         (V, variant, hairs, items, nitems, grid, defer_sdf, stream))
+          int defer_sdf, hipStream_t stream, hipEvent_t t0, hipEvent_t t1),
+         (V, variant, hairs, items, nitems, grid, defer_sdf, stream, t0, t1))
 TFX_DUAL(hipError_t, launch_collide_items,
-         (const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, hipStream_t stream),
-         (hairs, items, nitems, stream))
+         (const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, hipStream_t stream, hipEvent_t t0,
+          hipEvent_t t1),
+         (hairs, items, nitems, stream, t0, t1))
 TFX_DUAL(int, collide_chunk, (), ())
 TFX_DUAL(int, fused_block_vertices, (int V, int variant), (V, variant))
 TFX_DUAL(int, fused_occupancy, (int V, int variant), (V, variant))
@@ -123,20 +124,35 @@ inline hipEvent_t pool_event() {
 struct PassTimer {
     const char* label;
     hipStream_t s;
-    hipEvent_t a = nullptr;
-    PassTimer(const char* l, hipStream_t st) : label(l), s(st) {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool dispatch;
+    // in_dispatch: the pass is one kernel launched with start() and stop()
+    // (hipExtLaunchKernel), whose own begin and end stamp the pair: the
+    // kernel's duration without the gap before its dispatch.  Otherwise the
+    // pair is recorded on the stream around the pass.
+    PassTimer(const char* l, hipStream_t st, bool in_dispatch = false) : label(l), s(st), dispatch(in_dispatch) {
         if (!label) return;
         std::lock_guard<std::mutex> g(g_tmu);
         if (!g_timing) return;
         a = pool_event();
-        if (a) (void)hipEventRecord(a, s);
+        if (!a) return;
+        if (dispatch) {
+            b = pool_event();
+            if (!b) { g_event_pool.push_back(a); a = nullptr; }
+            return;
+        }
+        (void)hipEventRecord(a, s);
     }
+    hipEvent_t start() const { return dispatch ? a : nullptr; }
+    hipEvent_t stop() const { return dispatch ? b : nullptr; }
     ~PassTimer() {
         if (!a) return;
         std::lock_guard<std::mutex> g(g_tmu);
-        hipEvent_t b = pool_event();
-        if (!b) return;
-        (void)hipEventRecord(b, s);
+        if (!dispatch) {
+            b = pool_event();
+            if (!b) return;
+            (void)hipEventRecord(b, s);
+        }
         g_timers[label].push_back(TimerRec{a, b});
     }
 };

@@ -113,6 +113,7 @@ struct tfx_server {
     std::vector<hipEvent_t> event_pool;
     tfx_simulation shape;  // launch-shape policy of the fused launches
     std::vector<Instance*> graveyard;  // retired instances to free (worker only)
+    bool freeing = false;              // the worker is freeing a batch taken from the graveyard
     // statistics since the last reset
     std::vector<double> latencies;
     long long completed_requests = 0, completed_frames = 0, vertex_substeps = 0, passes = 0;
@@ -400,7 +401,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
             if (!cnt) continue;
             const int grid = std::min(cnt, shapes[V].resident);
             HIP_TRY(kl_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], cnt, grid,
-                                     0, st));
+                                     0, st, nullptr, nullptr));
         }
     HIP_TRY(hipEventRecord(pass.ev, st));
     if (!sl.ev) HIP_TRY(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
@@ -417,9 +418,15 @@ void worker_main(tfx_server* s) {
         if (!s->graveyard.empty()) {
             std::vector<Instance*> g;
             g.swap(s->graveyard);
+            s->freeing = true;
             lk.unlock();
             for (Instance* in : g) free_instance(in);
             lk.lock();
+            s->freeing = false;
+            s->cv_done.notify_all();
+            // instances retired while the lock was released: their
+            // notifications found no waiter, so look again before sleeping
+            continue;
         }
         // completed passes
         while (!s->inflight.empty() && hipEventQuery(s->inflight.front().ev) == hipSuccess) retire_pass(s);
@@ -450,6 +457,7 @@ void worker_main(tfx_server* s) {
                 if (e != hipSuccess) fail_all(s, TFX_ERR_HIP, std::string("pass failed: ") + hipGetErrorString(e));
                 continue;
             }
+            if (!s->graveyard.empty()) continue;  // retire_pass above may have added some
             if (s->stop) break;
             s->cv_done.notify_all();  // idle
             s->cv_work.wait(lk);
@@ -678,7 +686,12 @@ extern "C" int tfx_server_drain(tfx_server* s) {
     CHECK_ARG(s, "tfx_server_drain: null server");
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_work.notify_all();
-    s->cv_done.wait(lk, [&] { return (s->unfinished == 0 && s->inflight.empty()) || s->error != TFX_OK; });
+    // retired instances' memory is released too, so that frees (which
+    // synchronise the device) do not land in the work submitted next
+    s->cv_done.wait(lk, [&] {
+        return (s->unfinished == 0 && s->inflight.empty() && s->graveyard.empty() && !s->freeing) ||
+               s->error != TFX_OK;
+    });
     if (s->error) return fail(s->error, "tfx_server: %s", s->error_msg.c_str());
     return TFX_OK;
 }

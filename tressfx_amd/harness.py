@@ -393,6 +393,7 @@ class ServedRun:
         t1 = time.perf_counter() - t0
         for iid, _ in ids:
             srv.retire(iid)
+        srv.drain()  # their memory released before the caller's next measurement
         return {"window_s": t1, "submit_s": t_sub, "instances": len(ids), "frames_per_instance": int(frames),
                 **srv.stats()}
 
