@@ -605,6 +605,11 @@ int tfx_server_submit(tfx_server* server, int instance_id, int frames, const flo
  * latency_ms = completion - submission. */
 int tfx_server_poll(tfx_server* server, unsigned long long ticket, int* done, double* latency_ms);
 int tfx_server_wait(tfx_server* server, unsigned long long ticket, double* latency_ms);
+/* As tfx_batch_set_sdf_deferral, for the passes the server launches from now
+ * on.  The server's default is TFX_SDF_DEFER_NEVER: its passes are small (a
+ * few instances), where the collide pass's extra launch costs as much as it
+ * saves. */
+int tfx_server_set_sdf_deferral(tfx_server* server, int mode);
 /* Blocks until nothing is queued or running and the memory of retired
  * instances is released. */
 int tfx_server_drain(tfx_server* server);

@@ -73,8 +73,13 @@ def compare(a, b, what):
         assert mm is None, f"{what}: {k} differs at {mm}"
 
 
-def test_alone_in_mix_and_per_object_are_identical(gpu):
-    print("[server] alone vs mix vs per-object", flush=True)
+@pytest.mark.parametrize("mix_deferral", ["never", "auto"])
+def test_alone_in_mix_and_per_object_are_identical(gpu, mix_deferral):
+    """The alone runs keep the SDF collision inside the fused launches (the
+    server's default); the mix runs it there too (never) or, chosen per pass
+    (auto: these passes fit in one round of resident blocks), as a collide
+    pass after each substep's fused launches."""
+    print(f"[server] alone vs mix ({mix_deferral}) vs per-object", flush=True)
     x = make_instance(0, 900, 32)
     y = make_instance(1, 400, 16)
     others = [make_instance(2 + k, G, V) for k, (G, V) in enumerate([(1500, 16), (300, 64), (700, 32), (2000, 16)])]
@@ -90,6 +95,7 @@ def test_alone_in_mix_and_per_object_are_identical(gpu):
         srv.close()
 
     srv = tfx.HairServer(substeps_per_frame=SUBSTEPS, substep_dt=float(DT), max_inflight=2)
+    srv.set_sdf_deferral(tfx.TFX_SDF_DEFER_NEVER if mix_deferral == "never" else tfx.TFX_SDF_DEFER_AUTO)
     ids = [admit(srv, wl, mesh) for wl, mesh in others]
     xid, yid = admit(srv, *x), admit(srv, *y)
     pos = {i: 0 for i in ids}

@@ -368,6 +368,7 @@ EXPORTS = {
     "tfx_server_poll": (_I, [_VP, ctypes.c_ulonglong, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
     "tfx_server_wait": (_I, [_VP, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_double)]),
     "tfx_server_drain": (_I, [_VP]),
+    "tfx_server_set_sdf_deferral": (_I, [_VP, _I]),
     "tfx_server_stats_read": (_I, [_VP, ctypes.POINTER(ServerStats), _I]),
     "tfx_server_get_objects": (_I, [_VP, _I, ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "tfx_batch_create": (_I, [ctypes.POINTER(_VP)]),
@@ -993,6 +994,10 @@ class HairServer:
 
     def drain(self):
         _check(self._lib.tfx_server_drain(self.handle))
+
+    def set_sdf_deferral(self, mode):
+        """As HairBatch.set_sdf_deferral, for the passes launched from now on."""
+        _check(self._lib.tfx_server_set_sdf_deferral(self.handle, int(mode)))
 
     def stats(self, reset=False):
         st = ServerStats()

@@ -18,6 +18,7 @@
 #include <condition_variable>
 #include <deque>
 #include <thread>
+#include <atomic>
 #include <unordered_map>
 
 namespace {
@@ -114,6 +115,10 @@ struct tfx_server {
     tfx_simulation shape;  // launch-shape policy of the fused launches
     std::vector<Instance*> graveyard;  // retired instances to free (worker only)
     bool freeing = false;              // the worker is freeing a batch taken from the graveyard
+    // per pass and strand length, as the batched form; off by default: the
+    // served passes are small, and deferring gained nothing there (the extra
+    // launch per substep cost about as much as it saved)
+    std::atomic<int> defer_sdf{TFX_SDF_DEFER_NEVER};
     // statistics since the last reset
     std::vector<double> latencies;
     long long completed_requests = 0, completed_frames = 0, vertex_substeps = 0, passes = 0;
@@ -277,6 +282,31 @@ int launch_pass(tfx_server* s, Pass& pass) {
             for (int sub = 0; sub < pass.frames[m].second->substeps; ++sub) vcount[(size_t)V * subs + sub] = upto;
         }
     }
+    // SDF collision deferred to a collide pass after each substep's fused
+    // launches for the strand lengths whose blocks fit in one round of
+    // resident blocks (as the batched form); collide items (member, first
+    // vertex) ordered by member substeps, so substep `sub` runs a prefix
+    int defer[65] = {0};
+    const int dmode = s->defer_sdf.load();
+    for (int V = 4; V <= 64; V *= 2) {
+        const int cnt = vcount[(size_t)V * subs];
+        defer[V] = cnt > 0 && (dmode == TFX_SDF_DEFER_ALWAYS || (dmode == TFX_SDF_DEFER_AUTO && cnt <= shapes[V].resident));
+    }
+    const int cbegin = (int)items.size();
+    std::vector<int> ccount(subs, 0);
+    {
+        std::vector<int> cm;
+        for (int m = 0; m < n; ++m)
+            if (defer[hairs[m]->V] && pass.frames[m].second->sdf) cm.push_back(m);
+        std::stable_sort(cm.begin(), cm.end(), [&](int a, int b) {
+            return pass.frames[a].second->substeps > pass.frames[b].second->substeps;
+        });
+        const int chunk = kl_collide_chunk();
+        for (int m : cm) {
+            for (int v0 = 0; v0 < hairs[m]->N; v0 += chunk) items.push_back(tfx::WorkItem{m, v0});
+            for (int sub = 0; sub < pass.frames[m].second->substeps; ++sub) ccount[sub] = (int)items.size() - cbegin;
+        }
+    }
     const size_t total = off_items + items.size() * sizeof(tfx::WorkItem);
     // ---- slot
     Slot& sl = s->slots[s->next_slot];
@@ -395,14 +425,18 @@ int launch_pass(tfx_server* s, Pass& pass) {
     }
     const DevHair* dtables = reinterpret_cast<const DevHair*>(D + off_tables);
     const tfx::WorkItem* ditems = reinterpret_cast<const tfx::WorkItem*>(D + off_items);
-    for (int sub = 0; sub < subs; ++sub)
+    for (int sub = 0; sub < subs; ++sub) {
         for (int V = 4; V <= 64; V *= 2) {
             const int cnt = vcount[(size_t)V * subs + sub];
             if (!cnt) continue;
             const int grid = std::min(cnt, shapes[V].resident);
             HIP_TRY(kl_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], cnt, grid,
-                                     0, st, nullptr, nullptr));
+                                     defer[V], st, nullptr, nullptr));
         }
+        if (ccount[sub])
+            HIP_TRY(kl_launch_collide_items(dtables + (size_t)sub * n, ditems + cbegin, ccount[sub], st, nullptr,
+                                             nullptr));
+    }
     HIP_TRY(hipEventRecord(pass.ev, st));
     if (!sl.ev) HIP_TRY(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(sl.ev, st));
@@ -595,6 +629,14 @@ extern "C" int tfx_server_admit(tfx_server* s, const tfx_instance_desc* d, int* 
     in->id = s->next_id++;
     s->instances[in->id] = in;
     *id = in->id;
+    return TFX_OK;
+}
+
+extern "C" int tfx_server_set_sdf_deferral(tfx_server* s, int mode) {
+    CHECK_ARG(s, "tfx_server_set_sdf_deferral: null server");
+    CHECK_ARG(mode >= TFX_SDF_DEFER_AUTO && mode <= TFX_SDF_DEFER_ALWAYS,
+              "tfx_server_set_sdf_deferral: mode must be TFX_SDF_DEFER_AUTO, _NEVER or _ALWAYS");
+    s->defer_sdf.store(mode);  // passes launched from now on
     return TFX_OK;
 }
 
