@@ -338,6 +338,9 @@ int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, tfx_hair* c
  * resident blocks); `blocks_per_cu` caps the persistent grid per CU (0 = the
  * kernel's occupancy).  Results do not depend on the shape. */
 int tfx_simulation_set_launch_shape(tfx_simulation* sim, int vertices_per_strand, int variant, int blocks_per_cu);
+/* As tfx_batch_set_sdf_deferral, for tfx_simulation_simulate_fused (values
+ * TFX_SDF_DEFER_*, defined with the batched form below; default automatic). */
+int tfx_simulation_set_sdf_deferral(tfx_simulation* sim, int mode);
 
 /* ------------------------------------------------------------------------ */
 /* TressFXSDFCollisionSystem / TressFXSDFCollision                          */

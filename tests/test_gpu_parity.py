@@ -192,15 +192,19 @@ def test_sdf_reference_grid_sizing(gpu):
     assert bits_equal(g, ref)
 
 
-def test_collide_and_fused_vs_oracle(gpu):
+@pytest.mark.parametrize("deferral", ["auto", "never"])
+def test_collide_and_fused_vs_oracle(gpu, deferral):
     """Plain Simulate + CollideWithHair and the fused substep, stepwise against
     the FP32 oracle (bit-exact) and the FP64 oracle (1e-5), with SDF, wind and
-    follow strands."""
+    follow strands; the fused substep's SDF collision as a collide pass after
+    its launch (auto: this object fits in one round of resident blocks) or
+    inside it (never)."""
     torch = gpu
     wl, dm = _mesh_setup(torch)
     origin, cell, dims = wl.sdf_grid
     system = tfx.TressFXSDFCollisionSystem()
     sim = tfx.TressFXSimulation()
+    sim.set_sdf_deferral(tfx.TFX_SDF_DEFER_AUTO if deferral == "auto" else tfx.TFX_SDF_DEFER_NEVER)
     hair_a = tfx.TressFXHairObject(wl.asset)   # plain dispatch
     hair_b = tfx.TressFXHairObject(wl.asset)   # fused
     desc = dm.update(0.0)

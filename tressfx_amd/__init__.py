@@ -378,6 +378,7 @@ EXPORTS = {
     "tfx_batch_count": (_I, [_VP, ctypes.POINTER(_I)]),
     "tfx_batch_set_launch_shape": (_I, [_VP, _I, _I, _I]),
     "tfx_batch_set_sdf_deferral": (_I, [_VP, _I]),
+    "tfx_simulation_set_sdf_deferral": (_I, [_VP, _I]),
     "tfx_batch_autotune": (_I, [_VP, _VP, _I, _VP, _I, _VP]),
     "tfx_batch_begin_frame": (_I, [_VP]),
     "tfx_batch_stage_substep": (_I, [_VP, c_float_p, _I, ctypes.POINTER(SimulationSettings), _I, _F, c_float_p]),
@@ -646,6 +647,11 @@ class TressFXSimulation:
 
     def SetLaunchShape(self, vertices_per_strand, variant, blocks_per_cu=0):
         _check(self._lib.tfx_simulation_set_launch_shape(self.handle, vertices_per_strand, variant, blocks_per_cu))
+
+    def set_sdf_deferral(self, mode):
+        """TFX_SDF_DEFER_*: SDF collision inside SimulateFused's fused launches
+        or as a collide pass after them (same results)."""
+        _check(self._lib.tfx_simulation_set_sdf_deferral(self.handle, int(mode)))
 
     def SimulateFused(self, hair_objects, sdfs=None, stream=None):
         n = len(hair_objects)

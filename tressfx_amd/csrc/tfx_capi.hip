@@ -444,6 +444,14 @@ extern "C" int tfx_simulation_simulate(tfx_simulation* sim, void* stream, tfx_ha
     return TFX_OK;
 }
 
+extern "C" int tfx_simulation_set_sdf_deferral(tfx_simulation* sim, int mode) {
+    CHECK_ARG(sim, "tfx_simulation_set_sdf_deferral: null simulation");
+    CHECK_ARG(mode >= TFX_SDF_DEFER_AUTO && mode <= TFX_SDF_DEFER_ALWAYS,
+              "tfx_simulation_set_sdf_deferral: mode must be TFX_SDF_DEFER_AUTO, _NEVER or _ALWAYS");
+    sim->defer_sdf = mode;
+    return TFX_OK;
+}
+
 extern "C" int tfx_simulation_set_launch_shape(tfx_simulation* sim, int V, int variant, int blocks_per_cu) {
     CHECK_ARG(sim && kl_fused_block_vertices(V, 0) > 0, "tfx_simulation_set_launch_shape: bad vertices per strand");
     CHECK_ARG(variant >= -1 && variant <= 2, "tfx_simulation_set_launch_shape: variant must be -1 (automatic), 0, 1 or 2");
@@ -497,6 +505,20 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
                 for (int g0 = 0; g0 < hairs[i]->G; g0 += B) items.push_back(tfx::WorkItem{i, g0});
         vclass_count[V] = (int)items.size() - vclass_begin[V];
     }
+    // SDF collision as a collide pass after the fused launches for the strand
+    // lengths whose blocks fit in one round of resident blocks (as the
+    // batched form, tfx_simulation_set_sdf_deferral)
+    std::vector<int> defer(65, 0);
+    for (int V = 4; V <= 64; V *= 2) {
+        const int cnt = vclass_count[V], mode = sim->defer_sdf;
+        defer[V] = cnt > 0 && (mode == TFX_SDF_DEFER_ALWAYS || (mode == TFX_SDF_DEFER_AUTO && cnt <= vclass_resident[V]));
+    }
+    const int cbegin = (int)items.size();
+    const int chunk = kl_collide_chunk();
+    for (int i = 0; i < n; ++i)
+        if (defer[hairs[i]->V] && sdfs && sdfs[i])
+            for (int v0 = 0; v0 < hairs[i]->N; v0 += chunk) items.push_back(tfx::WorkItem{i, v0});
+    const int ccount = (int)items.size() - cbegin;
     const size_t hair_bytes = (size_t)n * sizeof(DevHair);
     const size_t item_off = (hair_bytes + 255) & ~size_t(255);
     const size_t bytes = item_off + items.size() * sizeof(tfx::WorkItem);
@@ -518,8 +540,12 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         if (!cnt) continue;
         const int grid = std::min(cnt, vclass_resident[V]);
         PassTimer tm("FusedSubstep", s, true);
-        HIP_TRY(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, 0, s, tm.start(),
-                                 tm.stop()));
+        HIP_TRY(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, defer[V], s,
+                                 tm.start(), tm.stop()));
+    }
+    if (ccount) {
+        PassTimer tm("CollideHairVerticesWithSdf", s, true);
+        HIP_TRY(kl_launch_collide_items(dtab, ditems + cbegin, ccount, s, tm.start(), tm.stop()));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
     return TFX_OK;
