@@ -56,6 +56,11 @@ struct tfx_batch {
     int exec_substeps[2] = {0, 0};     // substeps captured in exec[k]
     int exec_tris[2] = {0, 0};         // SDF construct grid (largest mesh) captured in exec[k]
     int exec_arith[2] = {0, 0};        // kernel build (TFX_ARITH_*) captured in exec[k]
+
+    // SDF build beside the first substep's deferring fused launches (which
+    // read no SDF): a second stream forked from and joined to the frame's
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 static void release_device(tfx_batch* b) {
@@ -211,6 +216,13 @@ extern "C" int tfx_batch_create(tfx_batch** out) {
     b->device = g_device;
     b->shape.device = g_device;
     for (int v = 0; v <= 64; ++v) b->shape.variant[v] = b->shape.per_cu[v] = -1;
+    // created here, not inside a frame that may be under graph capture
+    if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+        tfx_batch_destroy(b);
+        return fail(TFX_ERR_HIP, "tfx_batch_create: stream or event creation failed");
+    }
     *out = b;
     return TFX_OK;
 }
@@ -220,6 +232,9 @@ extern "C" int tfx_batch_destroy(tfx_batch* b) {
     (void)hipDeviceSynchronize();
     release_device(b);
     release_host(b);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     delete b;
     return TFX_OK;
 }
@@ -347,19 +362,42 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
     HIP_TRY(hipMemcpyAsync(b->dev_slot, b->host[b->slot], b->off_sub + (size_t)substeps * b->sub_bytes,
                            hipMemcpyHostToDevice, s));
     const int nsdf = (int)b->sdf_members.size();
+    bool any_defer = false;
+    for (int V = 4; V <= 64; V *= 2) any_defer = any_defer || (b->vcount[V] && b->vdefer[V]);
+    const bool overlap = nsdf && any_defer;
+    hipStream_t sb = s;  // stream of the SDF build
+    if (overlap) {
+        HIP_TRY(hipEventRecord(b->fork, s));
+        HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0));
+        sb = b->side;
+    }
     if (nsdf) {
         const tfx::SdfJob* jobs = reinterpret_cast<const tfx::SdfJob*>(b->dev_slot);
         static const char* kNames[4] = {"InitializeSignedDistanceField", "ConstructSignedDistanceField",
                                         "FinalizeSignedDistanceField", "FinalizeSignedDistanceField"};
         for (int phase = 0; phase < 4; ++phase) {
-            PassTimer tm(timers ? kNames[phase] : nullptr, s);
-            HIP_TRY(kl_launch_sdf_multi(phase, jobs, nsdf, b->max_cells, b->max_tris, s));
+            PassTimer tm(timers ? kNames[phase] : nullptr, sb);
+            HIP_TRY(kl_launch_sdf_multi(phase, jobs, nsdf, b->max_cells, b->max_tris, sb));
         }
     }
+    if (overlap) HIP_TRY(hipEventRecord(b->join, b->side));
     for (int sub = 0; sub < substeps; ++sub) {
         const DevHair* tables = b->dev_tables + (size_t)sub * n;
+        // substep 0: the deferring classes first, beside the SDF build; the
+        // build joins before any launch that reads the SDF
+        if (overlap && sub == 0) {
+            for (int V = 4; V <= 64; V *= 2) {
+                if (!b->vcount[V] || !b->vdefer[V]) continue;
+                const int grid = std::min(b->vcount[V], b->vresident[V]);
+                PassTimer tm(timers ? "FusedSubstep" : nullptr, s, true);
+                HIP_TRY(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V], b->vcount[V], grid,
+                                         1, s, tm.start(), tm.stop()));
+            }
+            HIP_TRY(hipStreamWaitEvent(s, b->join, 0));
+        }
         for (int V = 4; V <= 64; V *= 2) {
             if (!b->vcount[V]) continue;
+            if (overlap && sub == 0 && b->vdefer[V]) continue;
             const int grid = std::min(b->vcount[V], b->vresident[V]);
             PassTimer tm(timers ? "FusedSubstep" : nullptr, s, true);
             HIP_TRY(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V], b->vcount[V], grid,
