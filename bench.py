@@ -504,7 +504,8 @@ def main():
                         f"{wl.substeps} substeps/frame, length iterations 2; step = one frame",
             "guide_strands": G, "follow_per_guide": F, "vertices_per_strand": V, "substeps": wl.substeps,
             "path": "batched form, one member: per frame one hipGraph replay = parameter upload, "
-                    "SDF build (4 launches), per substep a fused launch and a collide pass",
+                    "SDF build (4 launches) beside the first substep's fused launch, per substep a fused launch "
+                    "and a collide pass",
         },
         "latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99)),
                        "unit": "GPU ms per frame"},
