@@ -142,6 +142,32 @@ def test_batch_sdf_deferral_modes_equal_per_object_dispatch(gpu):
         batches[1].set_sdf_deferral(2)
 
 
+def test_batch_zero_substep_frame_joins_the_sdf_build(gpu):
+    """With a deferring class the batch builds the SDFs on a second stream that
+    the first substep joins; a frame with no substeps must join it too (a
+    graph capture would otherwise end with unjoined work, and a direct frame
+    could let the next frame's upload race the build).  Frames of 2, 0, 2
+    substeps, direct and graph-replayed, automatic deferral (which these
+    small classes take): hair state and every SDF equal the per-object
+    dispatch after each frame."""
+    torch = gpu
+    print("[batch] zero-substep frame with the overlapped SDF build", flush=True)
+    members = [Member(torch, i, *spec, copies=3) for i, spec in enumerate(MEMBERS)]
+    side = torch.cuda.Stream()
+    counts = [2, 0, 2, 0]
+    batches = None
+    for f in range(len(counts)):
+        batches = _run(torch, members, [f], lambda fr: counts[fr], plain_copy=0, batch_copies=(1, 2),
+                       batches=batches, stream_of=lambda c: side.cuda_stream, graph_of=lambda c: c == 2)
+        torch.cuda.synchronize()
+        for i, m in enumerate(members):
+            if m.sdfs[0] is None:
+                continue
+            want = m.sdfs[0].download().view(np.uint32)
+            for c in (1, 2):
+                assert np.array_equal(m.sdfs[c].download().view(np.uint32), want), f"frame {f} member {i} copy {c}"
+
+
 def test_batch_graph_recaptures_on_substep_change(gpu):
     """A frame with another substep count re-captures its slot's graph; both
     slots alternate between frames."""

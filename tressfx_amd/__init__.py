@@ -601,14 +601,16 @@ class TressFXHairObject:
         """The barrier back (TressFXHairObject.cpp:571-583)."""
         _check(self._lib.tfx_hair_transition_rendering_to_sim(self.handle, from_stream, to_stream))
 
-    def download_state(self, stream=None):
+    def download_state(self, stream=None, keys=("pos", "prev", "prevprev", "tangent", "strand_level")):
+        """Host copies of the dynamic state arrays named in `keys`."""
         n = self.GetNumTotalHairVertices()
         s = self.GetNumTotalHairStrands()
-        st = {k: np.zeros((n, 4), np.float32) for k in ("pos", "prev", "prevprev", "tangent")}
-        st["strand_level"] = np.zeros((s, 12), np.float32)
-        _check(self._lib.tfx_hair_download_state(self.handle, stream, _fptr(st["pos"]), _fptr(st["prev"]),
-                                                 _fptr(st["prevprev"]), _fptr(st["tangent"]),
-                                                 _fptr(st["strand_level"])))
+        st = {k: np.zeros((n, 4), np.float32) for k in ("pos", "prev", "prevprev", "tangent") if k in keys}
+        if "strand_level" in keys:
+            st["strand_level"] = np.zeros((s, 12), np.float32)
+        p = lambda k: _fptr(st[k]) if k in st else None
+        _check(self._lib.tfx_hair_download_state(self.handle, stream, p("pos"), p("prev"), p("prevprev"),
+                                                 p("tangent"), p("strand_level")))
         return st
 
     def upload_state(self, state, stream=None):

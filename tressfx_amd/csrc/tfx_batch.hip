@@ -390,8 +390,8 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
                 if (!b->vcount[V] || !b->vdefer[V]) continue;
                 const int grid = std::min(b->vcount[V], b->vresident[V]);
                 PassTimer tm(timers ? "FusedSubstep" : nullptr, s, true);
-                HIP_TRY(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V], b->vcount[V], grid,
-                                         1, s, tm.start(), tm.stop()));
+                HIP_TRY(tm.launched(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V],
+                                                    b->vcount[V], grid, 1, s, tm.start(), tm.stop())));
             }
             HIP_TRY(hipStreamWaitEvent(s, b->join, 0));
         }
@@ -400,14 +400,17 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
             if (overlap && sub == 0 && b->vdefer[V]) continue;
             const int grid = std::min(b->vcount[V], b->vresident[V]);
             PassTimer tm(timers ? "FusedSubstep" : nullptr, s, true);
-            HIP_TRY(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V], b->vcount[V], grid,
-                                     b->vdefer[V], s, tm.start(), tm.stop()));
+            HIP_TRY(tm.launched(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V],
+                                                b->vcount[V], grid, b->vdefer[V], s, tm.start(), tm.stop())));
         }
         if (b->ccount) {
             PassTimer tm(timers ? "CollideHairVerticesWithSdf" : nullptr, s, true);
-            HIP_TRY(kl_launch_collide_items(tables, b->dev_items + b->cbegin, b->ccount, s, tm.start(), tm.stop()));
+            HIP_TRY(tm.launched(
+                kl_launch_collide_items(tables, b->dev_items + b->cbegin, b->ccount, s, tm.start(), tm.stop())));
         }
     }
+    // A frame without substeps still joins the forked build (substep 0 joins it otherwise).
+    if (overlap && substeps == 0) HIP_TRY(hipStreamWaitEvent(s, b->join, 0));
     return TFX_OK;
 }
 

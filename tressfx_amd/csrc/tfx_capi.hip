@@ -540,12 +540,12 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         if (!cnt) continue;
         const int grid = std::min(cnt, vclass_resident[V]);
         PassTimer tm("FusedSubstep", s, true);
-        HIP_TRY(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid, defer[V], s,
-                                 tm.start(), tm.stop()));
+        HIP_TRY(tm.launched(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid,
+                                            defer[V], s, tm.start(), tm.stop())));
     }
     if (ccount) {
         PassTimer tm("CollideHairVerticesWithSdf", s, true);
-        HIP_TRY(kl_launch_collide_items(dtab, ditems + cbegin, ccount, s, tm.start(), tm.stop()));
+        HIP_TRY(tm.launched(kl_launch_collide_items(dtab, ditems + cbegin, ccount, s, tm.start(), tm.stop())));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
     return TFX_OK;

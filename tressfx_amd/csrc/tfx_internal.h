@@ -126,10 +126,13 @@ struct PassTimer {
     hipStream_t s;
     hipEvent_t a = nullptr, b = nullptr;
     bool dispatch;
+    bool stamped = false;
     // in_dispatch: the pass is one kernel launched with start() and stop()
     // (hipExtLaunchKernel), whose own begin and end stamp the pair: the
-    // kernel's duration without the gap before its dispatch.  Otherwise the
-    // pair is recorded on the stream around the pass.
+    // kernel's duration without the gap before its dispatch; the launch's
+    // status goes through launched(), and a pair whose launch failed (never
+    // stamped) is returned to the pool instead of being recorded.  Otherwise
+    // the pair is recorded on the stream around the pass.
     PassTimer(const char* l, hipStream_t st, bool in_dispatch = false) : label(l), s(st), dispatch(in_dispatch) {
         if (!label) return;
         std::lock_guard<std::mutex> g(g_tmu);
@@ -145,9 +148,18 @@ struct PassTimer {
     }
     hipEvent_t start() const { return dispatch ? a : nullptr; }
     hipEvent_t stop() const { return dispatch ? b : nullptr; }
+    hipError_t launched(hipError_t e) {
+        stamped = e == hipSuccess;
+        return e;
+    }
     ~PassTimer() {
         if (!a) return;
         std::lock_guard<std::mutex> g(g_tmu);
+        if (dispatch && !stamped) {
+            g_event_pool.push_back(a);
+            if (b) g_event_pool.push_back(b);
+            return;
+        }
         if (!dispatch) {
             b = pool_event();
             if (!b) return;

@@ -494,6 +494,9 @@ class ServedRun:
             "offered_request_rate": submitted / t_sub if t_sub > 0 else 0.0, "instances_retired": retired,
             "admissions_refused": refused, "resident_at_end": len(resident), **st,
         }
+        # instances still resident: workload index and frames run (inputs frames 0..n-1 of that
+        # workload while n stays within the horizon)
+        self.resident = {int(iid): (int(info[iid][0]), int(info[iid][1])) for iid in resident}
         if tenant_ids:
             tl = []
             for j, iid in enumerate(tenant_ids):

@@ -465,14 +465,15 @@ def _c3_asset_job(args):
     return config3_asset(*args)
 
 
-def config3(num_instances=256, seed=0, workers=1, **kw):
-    """The batched workload; assets are generated by `workers` forked
-    processes (call before the process touches the GPU)."""
+def config3(num_instances=256, seed=0, workers=1, start_method="fork", **kw):
+    """The batched workload; assets are generated by `workers` processes
+    started with `start_method` ("fork" only before the process touches the
+    GPU; "spawn" afterwards)."""
     spec = config3_spec(num_instances, seed, **kw)
     jobs = [(i, G, V, seed) for i, (G, V, n) in enumerate(spec)]
     if workers > 1:
         import multiprocessing as mp
-        with mp.get_context("fork").Pool(workers) as pool:
+        with mp.get_context(start_method).Pool(workers) as pool:
             assets = pool.map(_c3_asset_job, jobs, chunksize=1)
     else:
         assets = [config3_asset(*j) for j in jobs]
