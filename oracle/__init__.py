@@ -97,9 +97,10 @@ class OracleHair:
         self.strand_level = np.zeros((self.G * (self.F + 1), 12), dt)
 
     def set_state(self, st):
+        """Copies of the given arrays (the caller's arrays are never advanced in place)."""
         for k in ("pos", "prev", "prevprev", "tangent", "strand_level"):
             if k in st and st[k] is not None:
-                setattr(self, k, np.ascontiguousarray(st[k], self.dtype))
+                setattr(self, k, np.array(st[k], self.dtype, order="C", copy=True))
 
     def state(self):
         return {k: getattr(self, k).copy() for k in ("pos", "prev", "prevprev", "tangent", "strand_level")}
