@@ -22,15 +22,19 @@ SOURCES = [
     "tfx_asset.hip",
 ]
 
-# Kernel sources built a second time with the hardware reciprocal and square
-# root (TFX_ARITH_FAST): their kernels go to namespace tfx::fast and their C
-# entry points get a _fast suffix (tfx_device.h).
+# Kernel sources built a second time for TFX_ARITH_FAST: their kernels go to
+# namespace tfx::fast and their C entry points get a _fast suffix
+# (tfx_device.h).  That build divides by the hardware reciprocal (a / b =
+# a * v_rcp(b)), takes square roots and reciprocal square roots with the
+# hardware instructions (-fapprox-func: no range scaling around them), and
+# fuses multiply-adds (-ffp-contract=fast, overriding the exact build's off).
 FAST_SOURCES = [
     "tfx_sim_kernels.hip",
     "tfx_fused_kernel.hip",
     "tfx_sdf_kernels.hip",
 ]
-FAST_FLAGS = ["-DTFX_FAST_BUILD", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+FAST_FLAGS = ["-DTFX_FAST_BUILD", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fapprox-func",
+              "-ffp-contract=fast"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
