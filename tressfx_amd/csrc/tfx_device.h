@@ -205,6 +205,33 @@ __device__ __forceinline__ float4 quat_two_vectors(f3 u, f3 v) {
     return normalize_quat(make_float4(n.x, n.y, n.z, r));
 }
 
+// MultQuaternionAndVector(QuatFromTwoUnitVectors(u, v), x): x rotated by the
+// rotation taking u onto v, the step of LocalShapeConstraints :808-814.  The
+// fast build rotates by the unnormalised quaternion q = (u x v, 1 + u.v)
+// directly, x + (2 / |q|^2) (q.w (q.xyz x x) + q.xyz x (q.xyz x x)), without
+// normalising it first; both branches of QuatFromTwoUnitVectors and the
+// near-zero case of NormalizeQuaternion :329-342 keep their meaning.
+__device__ __forceinline__ f3 rotate_from_to(f3 u, f3 v, f3 x) {
+#ifndef TFX_FAST_BUILD
+    return qrot(quat_two_vectors(u, v), x);
+#else
+    float r = 1.0f + dot3(u, v);
+    f3 c;
+    if (r < 1e-7f) {
+        r = 0.0f;
+        c = fabsf(u.x) > fabsf(u.z) ? f3{-u.y, u.x, 0.0f} : f3{0.0f, -u.z, u.y};
+    } else {
+        c = cross3(u, v);
+    }
+    const float qq = dot3(c, c) + r * r;
+    if (qq < 1e-10f) return qrot(make_float4(c.x, c.y, c.z, 1.0f), x);
+    const float s = 2.0f / qq;
+    const f3 t = cross3(c, x);
+    const f3 ct = cross3(c, t);
+    return add(x, mul(s, add(mul(r, t), ct)));
+#endif
+}
+
 // Blended bone of one strand: ApplyVertexBoneSkinning's matrix path
 // (:549-567) — 3x3 rotation/scale part plus translation row.
 struct BlendedBone {
@@ -245,6 +272,36 @@ __device__ __forceinline__ void blend_bones(const tfx_bone_skinning_data& sd, co
         if (wi > 0.0f) {
 #pragma unroll
             for (int k = 0; k < 12; ++k) acc[k] = acc[k] + mat[i][k] * wi;
+            wsum = wsum + wi;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b.m[k] = acc[k] / wsum;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) b.t[k] = acc[9 + k] / wsum;
+}
+
+// blend_bones with the matrices read one after another into the running sum
+// (same operations in the same order, a quarter of the registers): for
+// callers with other state live.
+__device__ __forceinline__ void blend_bones_lean(const tfx_bone_skinning_data& sd, const float* bones, BlendedBone& b) {
+    const float* m0 = bone_ptr(bones, sd.bone_index[0]);
+    const float w0 = sd.bone_weight[0];
+    float acc[12];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[r * 3 + c] = ldg(&m0[4 * r + c]) * w0;
+    float wsum = w0;
+#pragma unroll 1
+    for (int i = 1; i < 4; ++i) {
+        const float wi = sd.bone_weight[i];
+        if (wi > 0.0f) {
+            const float* mi = bone_ptr(bones, sd.bone_index[i]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[r * 3 + c] = acc[r * 3 + c] + ldg(&mi[4 * r + c]) * wi;
             wsum = wsum + wi;
         }
     }

@@ -21,6 +21,8 @@
 // only (the only ones any stage reads) and StrandLevelData stays on chip.
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "tfx_device.h"
 
 namespace tfx {
@@ -39,8 +41,10 @@ struct FusedShape {
     static_assert(B >= 1 && B * V == VB, "VB must be a multiple of V");
 };
 
+// Occupancy target: 4 blocks per CU for 512-vertex blocks, 3 for larger ones
+// (the register budget of 128 and 168 VGPRs per lane).
 template <int V, int VB>
-__global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair* __restrict__ hairs,
+__global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_substep(const DevHair* __restrict__ hairs,
                                                                const WorkItem* __restrict__ items, int nitems,
                                                                int defer_sdf) {
     using S = FusedShape<V, VB>;
@@ -64,6 +68,26 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
     // wave offset, so that blocks with B <= 64 do not all load SIMD 0.
     const int tt = (t + 64 * (blockIdx.x & 3)) & (kFusedBlock - 1);
 
+    // P0 of a work item: the blended bone and skinning quaternion of each of
+    // its strands, threads first, first + count, ... taking one strand each.
+    // `lean`: with the item's vertex registers live (during P4).
+    auto blend_strands = [&](const DevHair& hh, int gg0, int nnb, int first, int count, auto lean) {
+        for (int s = first; s < nnb; s += count) {
+            const size_t strand = (size_t)(gg0 + s) * (hh.F + 1);
+            BlendedBone bb;
+            if constexpr (decltype(lean)::value) blend_bones_lean(ldg(&hh.skin[strand]), hh.bones, bb);
+            else blend_bones(ldg(&hh.skin[strand]), hh.bones, bb);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) sMat[s][k] = bb.m[k];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sMat[s][9 + k] = bb.t[k];
+            sSkinQ[s] = quat_from_matrix3(bb.m);
+        }
+    };
+    // Whether the current item's P0 was done ahead, during the previous
+    // item's LocalShapeConstraints phase.
+    bool blended = false;
+
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const WorkItem item = items[w];
         const int hi = __builtin_amdgcn_readfirstlane(item.hair);
@@ -85,18 +109,12 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
             for (int k = t; k < brick_words; k += kFusedBlock) sBits[k] = ldg(&h.sdf_brick_bits[k]);
 
         // ---- P0: blended bone per strand ---------------------------------
-        for (int s = tt; s < nb; s += kFusedBlock) {
-            const size_t strand = (size_t)(g0 + s) * (F + 1);
-            BlendedBone bb;
-            blend_bones(ldg(&h.skin[strand]), h.bones, bb);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) sMat[s][k] = bb.m[k];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) sMat[s][9 + k] = bb.t[k];
-            sSkinQ[s] = quat_from_matrix3(bb.m);
-            if (F <= kMaxStagedFollow)
+        if (!blended) blend_strands(h, g0, nb, tt, kFusedBlock, std::false_type{});
+        if (F > 0 && F <= kMaxStagedFollow)
+            for (int s = tt; s < nb; s += kFusedBlock) {
+                const size_t strand = (size_t)(g0 + s) * (F + 1);
                 for (int j = 0; j < F; ++j) sOff[s][j] = ldg(&h.follow_off[strand + 1 + j]);
-        }
+            }
         __syncthreads();
 
         // ---- P1: integration and global shape ------------------------------
@@ -225,14 +243,26 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
         // order gives it.  The lanes of one strand are adjacent in one wave;
         // the wave runs the steps in lock step and orders its LDS accesses by
         // a wave-scope fence per step.
-        if (P.lsc_passes > 0) {
+        // The waves the sweep leaves idle compute P0 of the block's next work
+        // item meanwhile (sMat and sSkinQ are dead from P4 on: P1 last read
+        // them, and P2/P3 their VSP reuse).
+        constexpr int kLscPipe = 4, kLscLag = 2;
+        const int npass = P.lsc_passes > 0 ? P.lsc_passes : 0;
+        // 1, 2 or 4 lanes per strand: a count that divides 64 keeps the
+        // lanes of a strand in one wave (with 3 or more passes the extra
+        // lanes of a group are idle).
+        const int lanes = npass >= 3 ? kLscPipe : npass;
+        const int lsc_threads = min(((nb * lanes + 63) >> 6) << 6, kFusedBlock);
+        const int wn = w + (int)gridDim.x;
+        const bool ahead = wn < nitems && lsc_threads < kFusedBlock;
+        if (ahead && tt >= lsc_threads) {
+            const WorkItem nx = items[wn];
+            const DevHair hn = load_uniform(hairs + __builtin_amdgcn_readfirstlane(nx.hair));
+            const int gn = __builtin_amdgcn_readfirstlane(nx.g0);
+            blend_strands(hn, gn, min(B, hn.G - gn), tt - lsc_threads, kFusedBlock - lsc_threads, std::true_type{});
+        }
+        if (npass > 0) {
             const float stiff = 0.5f * fmn(P.local_stiffness, 0.95f);
-            constexpr int kLscPipe = 4, kLscLag = 2;
-            const int npass = P.lsc_passes;
-            // 1, 2 or 4 lanes per strand: a count that divides 64 keeps the
-            // lanes of a strand in one wave (with 3 or more passes the extra
-            // lanes of a group are idle).
-            const int lanes = npass >= 3 ? kLscPipe : npass;
             for (int item = tt; item < nb * lanes; item += kFusedBlock) {
                 const int s = item / lanes, j = item - s * lanes;
                 float4* sp = &sPos[s * STRIDE];
@@ -256,8 +286,7 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                             const f3 vecBindPose = xyz(vbw);
                             const f3 nLastBind{vbw.w, nyz.x, nyz.y};
                             f3 lastVec = sub(xyz(p), xyz(pm));
-                            float4 rg = quat_two_vectors(nLastBind, normalize3(lastVec));
-                            f3 org = add(qrot(rg, vecBindPose), xyz(p));
+                            f3 org = add(rotate_from_to(nLastBind, normalize3(lastVec), vecBindPose), xyz(p));
                             f3 del = mul(stiff, sub(org, xyz(pp)));
                             if (movable(p)) { p.x -= del.x; p.y -= del.y; p.z -= del.z; }
                             if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
@@ -272,8 +301,9 @@ __global__ void __launch_bounds__(kFusedBlock, 4) k_fused_substep(const DevHair*
                 }
                 __builtin_amdgcn_s_setprio(0);
             }
-            __syncthreads();
         }
+        blended = ahead;
+        __syncthreads();
 
         // ---- P5: wind, length constraints, capsules ------------------------
         if (P.wind_on) {

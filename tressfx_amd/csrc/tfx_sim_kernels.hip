@@ -128,8 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_lsc(DevHair h) {
         f3 lastVec = sub(xyz(pos), xyz(pm));
         f3 vecBindPose = sub(bindP, bind);
         f3 lastVecBindPose = sub(bind, bindM);
-        float4 rotGlobal = quat_two_vectors(normalize3(lastVecBindPose), normalize3(lastVec));
-        f3 org = add(qrot(rotGlobal, vecBindPose), xyz(pos));
+        f3 org = add(rotate_from_to(normalize3(lastVecBindPose), normalize3(lastVec), vecBindPose), xyz(pos));
         f3 del = mul(stiff, sub(org, xyz(pp)));
         if (movable(pos)) { pos.x -= del.x; pos.y -= del.y; pos.z -= del.z; }
         if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
