@@ -45,13 +45,40 @@ struct f3 { float x, y, z; };
 // of the object tables are generic, and generic (flat) accesses also count
 // against the LDS wait counter, so every wait for one of them waits for LDS
 // traffic too.
+// A 16-byte record (float4, uint4) is moved as a native 4-dword vector: the
+// copy constructor of HIP's vector types would take the record's address as
+// a generic pointer again and turn the access into a flat one.
 template <typename T>
 __device__ __forceinline__ T ldg(const T* p) {
-    return *(const TFX_ADDRSPACE(1) T*)p;
+    if constexpr (sizeof(T) == 16 && alignof(T) >= 16) {
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        const u4 v = *(const TFX_ADDRSPACE(1) u4*)p;
+        T out;
+        __builtin_memcpy(&out, &v, 16);
+        return out;
+    } else if constexpr (sizeof(T) > 16 && sizeof(T) % 4 == 0) {
+        // larger records (the 32-byte skinning record) word by word
+        const TFX_ADDRSPACE(1) unsigned int* w = (const TFX_ADDRSPACE(1) unsigned int*)p;
+        unsigned int words[sizeof(T) / 4];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 4); ++i) words[i] = w[i];
+        T out;
+        __builtin_memcpy(&out, words, sizeof(T));
+        return out;
+    } else {
+        return *(const TFX_ADDRSPACE(1) T*)p;
+    }
 }
 template <typename T>
 __device__ __forceinline__ void stg(T* p, const T& v) {
-    *(TFX_ADDRSPACE(1) T*)p = v;
+    if constexpr (sizeof(T) == 16 && alignof(T) >= 16) {
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        u4 w;
+        __builtin_memcpy(&w, &v, 16);
+        *(TFX_ADDRSPACE(1) u4*)p = w;
+    } else {
+        *(TFX_ADDRSPACE(1) T*)p = v;
+    }
 }
 // Host-written records a kernel only reads (object tables, per-substep
 // parameters): read through the constant address space from a wave-uniform

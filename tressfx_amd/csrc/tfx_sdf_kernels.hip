@@ -242,10 +242,14 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_pack_bricks(const uint8_t* __
 __device__ __forceinline__ void triangle_node_box(const tfx_standard_vertex* verts, const uint32_t* idx, int tri,
                                                   const SdfGrid& g, f3& x0, f3& x1, f3& x2, int (&lo)[3],
                                                   int (&hi)[3]) {
-    const uint32_t i0 = idx[3 * tri + 0], i1 = idx[3 * tri + 1], i2 = idx[3 * tri + 2];
-    x0 = f3{verts[i0].position[0], verts[i0].position[1], verts[i0].position[2]};
-    x1 = f3{verts[i1].position[0], verts[i1].position[1], verts[i1].position[2]};
-    x2 = f3{verts[i2].position[0], verts[i2].position[1], verts[i2].position[2]};
+    const uint32_t i0 = ldg(&idx[3 * tri + 0]), i1 = ldg(&idx[3 * tri + 1]), i2 = ldg(&idx[3 * tri + 2]);
+    auto vertex = [&](uint32_t i) {
+        const float* q = verts[i].position;
+        return f3{ldg(&q[0]), ldg(&q[1]), ldg(&q[2])};
+    };
+    x0 = vertex(i0);
+    x1 = vertex(i1);
+    x2 = vertex(i2);
     const float m = g.cell;  // MARGIN
     f3 mn{fmn(x0.x, fmn(x1.x, x2.x)) - m, fmn(x0.y, fmn(x1.y, x2.y)) - m, fmn(x0.z, fmn(x1.z, x2.z)) - m};
     f3 mx{fmx(x0.x, fmx(x1.x, x2.x)) + m, fmx(x0.y, fmx(x1.y, x2.y)) + m, fmx(x0.z, fmx(x1.z, x2.z)) + m};
@@ -300,8 +304,9 @@ __device__ __forceinline__ void construct_triangle(uint32_t* __restrict__ cells,
         const int x = lo[0] + xx, y = lo[1] + yy, z = lo[2] + zz;
         const float d = signed_distance(tp, sdf_cell_pos(g, x, y, z));
         const uint32_t f = float_flip3(d);
-        uint32_t* cell = &cells[sdf_index(g, x, y, z)];
-        if (f < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(cell, f);
+        TFX_ADDRSPACE(1) uint32_t* cell = (TFX_ADDRSPACE(1) uint32_t*)&cells[sdf_index(g, x, y, z)];
+        if (f < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            __hip_atomic_fetch_min(cell, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
