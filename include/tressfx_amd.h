@@ -203,9 +203,10 @@ const char* tfx_last_error(void);
  * GetDevice()->GetTimeStamp(name) after each dispatch (TressFXSimulation.cpp:87-106,
  * TressFXSDFCollision.cpp:130-209).  While enabled, HIP events bracket every
  * kernel launch; tfx_profiling_read waits for and sums the launches recorded
- * under `label` (a stage name, "FusedSubstep", "InitializeSignedDistanceField",
- * "ConstructSignedDistanceField", "FinalizeSignedDistanceField",
- * "CollideHairVerticesWithSdf") and forgets them. */
+ * under `label` or under any label extending it with '/' (a stage name,
+ * "FusedSubstep" -- recorded per strand length as "FusedSubstep/V16" etc. --,
+ * "InitializeSignedDistanceField", "ConstructSignedDistanceField",
+ * "FinalizeSignedDistanceField", "CollideHairVerticesWithSdf") and forgets them. */
 int tfx_profiling_enable(int enable);
 
 /* ---- Mesh-based strand transforms ---------------------------------------- */

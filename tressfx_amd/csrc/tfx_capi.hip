@@ -31,11 +31,21 @@ extern "C" int tfx_profiling_enable(int enable) {
 
 extern "C" int tfx_profiling_read(const char* label, double* total_ms, int* launches) {
     if (!label || !total_ms || !launches) return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_profiling_read: null argument");
+    // `label` itself and every label that extends it with '/' (the fused
+    // launches are recorded per strand length, "FusedSubstep/V16", ...).
     std::vector<TimerRec> recs;
     {
         std::lock_guard<std::mutex> g(g_tmu);
-        auto it = g_timers.find(label);
-        if (it != g_timers.end()) { recs.swap(it->second); g_timers.erase(it); }
+        const size_t n = std::strlen(label);
+        for (auto it = g_timers.begin(); it != g_timers.end();) {
+            const std::string& k = it->first;
+            if (k.compare(0, n, label) == 0 && (k.size() == n || k[n] == '/')) {
+                recs.insert(recs.end(), it->second.begin(), it->second.end());
+                it = g_timers.erase(it);
+            } else {
+                ++it;
+            }
+        }
     }
     double sum = 0.0;
     int rc = TFX_OK;
@@ -539,7 +549,7 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         const int cnt = vclass_count[V];
         if (!cnt) continue;
         const int grid = std::min(cnt, vclass_resident[V]);
-        PassTimer tm("FusedSubstep", s, true);
+        PassTimer tm(fused_label(V), s, true);
         HIP_TRY(tm.launched(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid,
                                             defer[V], s, tm.start(), tm.stop())));
     }

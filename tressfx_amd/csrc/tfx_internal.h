@@ -171,6 +171,18 @@ struct PassTimer {
 }  // namespace tfx_timing
 using tfx_timing::PassTimer;
 
+// Pass-timer label of the fused launches of strand length V.
+inline const char* fused_label(int V) {
+    switch (V) {
+        case 4: return "FusedSubstep/V4";
+        case 8: return "FusedSubstep/V8";
+        case 16: return "FusedSubstep/V16";
+        case 32: return "FusedSubstep/V32";
+        case 64: return "FusedSubstep/V64";
+        default: return "FusedSubstep";
+    }
+}
+
 template <class T> inline hipError_t dalloc(T** p, size_t count) {
     *p = nullptr;
     if (count == 0) return hipSuccess;

@@ -26,11 +26,11 @@ import os
 import statistics
 import sys
 
-FUSED_SOURCES = ("csrc/tfx_fused_kernel.hip", "csrc/tfx_device.h")
+FUSED_SOURCES = ("csrc/tfx_fused_kernel.hip", "csrc/tfx_device.h", "build.py")
 
 
 def fused_source_digest():
-    """sha256 over the sources that define k_fused_substep."""
+    """sha256 over the sources and build flags that define k_fused_substep."""
     h = hashlib.sha256()
     here = os.path.dirname(os.path.abspath(__file__))
     for rel in FUSED_SOURCES:
