@@ -333,6 +333,21 @@ int tfx_simulation_run_stage(tfx_simulation* sim, void* stream, tfx_hair* hair, 
  * maintained only for vertices 0 and 1 and StrandLevelData is not written. */
 int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, tfx_hair* const* hairs,
                                   tfx_sdf* const* sdfs, int num_hairs);
+/* tfx_simulation_simulate_fused without any allocation, on the host or the
+ * device.  No counterpart in the reference: it is for callers that cannot
+ * allocate -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same results as tfx_simulation_simulate_fused, with
+ * the SDF collision always inside the fused launches (no collide pass).  Its
+ * tables (parameter blocks, object rows, bone matrices, work list) live in a
+ * static device array of about 0.7 MiB per GPU, held until the process ends
+ * and shared by every call on that GPU: calls that overlap on one GPU race,
+ * and the array keeps the last call's contents.  Limits per call: 64 objects,
+ * 2048 bone matrices (the matrices each object's skinning records can read),
+ * 65,536 strand blocks; larger calls fail with TFX_ERR_INVALID_ARGUMENT.
+ * tfx_init resolves the array's device address (else the first call does,
+ * with the device current).  No pass timers. */
+int tfx_simulation_simulate_fused_noalloc(tfx_simulation* sim, void* stream, tfx_hair* const* hairs,
+                                          tfx_sdf* const* sdfs, int num_hairs);
 /* Launch shape of the fused kernel for strands of `vertices_per_strand`
  * vertices: `variant` 0, 1, 2 = strand blocks of 512, 1024, 2048 guide
  * vertices (at most 128 strands), -1 = chosen per call (fewest rounds of

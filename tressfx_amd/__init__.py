@@ -328,6 +328,7 @@ EXPORTS = {
     "tfx_simulation_simulate": (_I, [_VP, _VP, ctypes.POINTER(_VP), _I]),
     "tfx_simulation_run_stage": (_I, [_VP, _VP, _VP, _I]),
     "tfx_simulation_simulate_fused": (_I, [_VP, _VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I]),
+    "tfx_simulation_simulate_fused_noalloc": (_I, [_VP, _VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I]),
     "tfx_simulation_set_launch_shape": (_I, [_VP, _I, _I, _I]),
     "tfx_sdf_collision_system_create": (_I, [ctypes.POINTER(_VP)]),
     "tfx_sdf_collision_system_destroy": (_I, [_VP]),
@@ -662,6 +663,16 @@ class TressFXSimulation:
         if sdfs is not None:
             sarr = (ctypes.c_void_p * n)(*[(s.handle if s is not None else None) for s in sdfs])
         _check(self._lib.tfx_simulation_simulate_fused(self.handle, stream, arr, sarr, n))
+
+    def SimulateFusedNoAlloc(self, hair_objects, sdfs=None, stream=None):
+        """SimulateFused through tfx_simulation_simulate_fused_noalloc (no allocation; its limits are
+        64 objects, 2048 bone matrices and 65,536 strand blocks per call)."""
+        n = len(hair_objects)
+        arr = (ctypes.c_void_p * n)(*[h.handle for h in hair_objects])
+        sarr = None
+        if sdfs is not None:
+            sarr = (ctypes.c_void_p * n)(*[(s.handle if s is not None else None) for s in sdfs])
+        _check(self._lib.tfx_simulation_simulate_fused_noalloc(self.handle, stream, arr, sarr, n))
 
 
 class TressFXSDFCollisionSystem:

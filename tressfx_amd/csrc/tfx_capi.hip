@@ -130,13 +130,17 @@ extern "C" int tfx_get_version(int* major, int* minor, int* patch) {
     return TFX_OK;
 }
 
+static int noalloc_scratch(int device, void** out);
+
 extern "C" int tfx_init(int hip_device) {
     int n = 0;
     HIP_TRY(hipGetDeviceCount(&n));
     CHECK_ARG(hip_device >= 0 && hip_device < n, "tfx_init: no such HIP device");
     HIP_TRY(hipSetDevice(hip_device));
     g_device = hip_device;
-    return TFX_OK;
+    // the no-allocation path's scratch array on this device
+    void* scratch_ptr = nullptr;
+    return noalloc_scratch(hip_device, &scratch_ptr);
 }
 
 extern "C" int tfx_set_arithmetic(int mode) {
@@ -556,6 +560,162 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
     if (ccount) {
         PassTimer tm("CollideHairVerticesWithSdf", s, true);
         HIP_TRY(tm.launched(kl_launch_collide_items(dtab, ditems + cbegin, ccount, s, tm.start(), tm.stop())));
+    }
+    for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
+    return TFX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The fused substep without allocations (tfx_simulation_simulate_fused_noalloc)
+// ---------------------------------------------------------------------------
+// Its tables live in `scratch`, one array per GPU the library runs on, held
+// until the process ends: per call at most kNaMaxHairs objects, kNaMaxBones
+// bone matrices (the matrices each object's skinning can read) and
+// kNaMaxItems strand blocks.  The tables are written by staging launches
+// whose records travel as kernel arguments, so no host buffer has to outlive
+// the call and nothing is allocated on the host or the device.
+namespace {
+constexpr size_t na_align(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr int kNaMaxHairs = 64;
+constexpr int kNaMaxBones = 2048;
+constexpr int kNaMaxItems = 65536;
+constexpr int kNaStageBones = 24;  // matrices per staging launch (kernel arguments stay under 4 KiB)
+constexpr size_t kNaParamsOff = 0;
+constexpr size_t kNaHairsOff = na_align(kNaMaxHairs * sizeof(DevParams));
+constexpr size_t kNaBonesOff = kNaHairsOff + na_align(kNaMaxHairs * sizeof(DevHair));
+constexpr size_t kNaItemsOff = kNaBonesOff + na_align((size_t)kNaMaxBones * 16 * sizeof(float));
+constexpr size_t kNaScratchBytes = kNaItemsOff + (size_t)kNaMaxItems * sizeof(tfx::WorkItem);
+
+// One staging launch: the object's parameter block and table row (when
+// nitems > 0, its first launch), a chunk of its bone matrices, its work items.
+struct NaStage {
+    DevParams params;
+    DevHair hair;
+    int slot, bone_first, nbones, item_first, nitems, block_strands, pad0, pad1;
+    float bones[kNaStageBones * 16];
+};
+static_assert(sizeof(NaStage) <= 4000, "staging record must fit the kernel-argument space");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+__global__ void k_noalloc_stage(unsigned char* base, NaStage st) {
+    DevParams* params = reinterpret_cast<DevParams*>(base + kNaParamsOff);
+    DevHair* hairs = reinterpret_cast<DevHair*>(base + kNaHairsOff);
+    tfx::WorkItem* items = reinterpret_cast<tfx::WorkItem*>(base + kNaItemsOff);
+    float* bones = const_cast<float*>(st.hair.bones);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    if (st.nitems > 0 && t == 0) {
+        params[st.slot] = st.params;
+        hairs[st.slot] = st.hair;
+    }
+    for (int k = t; k < st.nbones * 16; k += stride) bones[st.bone_first * 16 + k] = st.bones[k];
+    for (int k = t; k < st.nitems; k += stride) items[st.item_first + k] = tfx::WorkItem{st.slot, k * st.block_strands};
+}
+
+// The device address of `scratch` on each device (the array's name in host
+// code is only a stand-in), looked up with the device current.
+static std::mutex g_na_mu;
+static void* g_na_scratch[64] = {nullptr};
+
+static int noalloc_scratch(int device, void** out) {
+    CHECK_ARG(device >= 0 && device < 64, "no-allocation path: device ordinal out of range");
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (!g_na_scratch[device]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        CHECK_ARG(cur == device, "no-allocation path: the object's device is not current (call tfx_init)");
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_na_scratch[device] = ptr;
+    }
+    *out = g_na_scratch[device];
+    return TFX_OK;
+}
+
+extern "C" int tfx_simulation_simulate_fused_noalloc(tfx_simulation* sim, void* stream, tfx_hair* const* hairs,
+                                                     tfx_sdf* const* sdfs, int n) {
+    CHECK_ARG(sim && (hairs || n == 0) && n >= 0, "tfx_simulation_simulate_fused_noalloc: bad argument");
+    if (n > kNaMaxHairs)
+        return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_simulation_simulate_fused_noalloc: %d hair objects (at most %d)", n,
+                    kNaMaxHairs);
+    if (n == 0) return TFX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int bones_total = 0;
+    for (int i = 0; i < n; ++i) {
+        CHECK_ARG(hairs[i], "tfx_simulation_simulate_fused_noalloc: null hair");
+        CHECK_ARG(hairs[i]->device == sim->device, "tfx_simulation_simulate_fused_noalloc: objects on another device");
+        if (sdfs && sdfs[i] && sdfs[i]->collide_variant != TFX_SDF_COLLIDE_FORWARD)
+            return fail(TFX_ERR_UNSUPPORTED, "tfx_simulation_simulate_fused_noalloc: the fused substep collides with "
+                                             "forward differences only (SDF %d has the mixed variant)", i);
+        bones_total += hairs[i]->bones_used;
+    }
+    if (bones_total > kNaMaxBones)
+        return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_simulation_simulate_fused_noalloc: %d bone matrices (at most %d)",
+                    bones_total, kNaMaxBones);
+    void* base = nullptr;
+    int rc = noalloc_scratch(sim->device, &base);
+    if (rc) return rc;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    // strand blocks per strand-length class, contiguous per class
+    int cls_begin[65] = {0}, cls_count[65] = {0}, cls_variant[65] = {0}, cls_resident[65] = {0}, cls_B[65] = {0};
+    int item_first[kNaMaxHairs], nitems[kNaMaxHairs];
+    const int cus = cu_count(sim->device);
+    int total = 0;
+    for (int V = 4; V <= 64; V *= 2) {
+        cls_begin[V] = total;
+        bool any = false;
+        for (int i = 0; i < n; ++i) any = any || hairs[i]->V == V;
+        if (!any) continue;
+        const Shape sh = choose_shape(sim, V, hairs, n, cus);
+        cls_variant[V] = sh.variant;
+        cls_resident[V] = sh.resident;
+        cls_B[V] = kl_fused_block_vertices(V, sh.variant) / V;
+        for (int i = 0; i < n; ++i)
+            if (hairs[i]->V == V) {
+                item_first[i] = total;
+                nitems[i] = (hairs[i]->G + cls_B[V] - 1) / cls_B[V];
+                total += nitems[i];
+            }
+        cls_count[V] = total - cls_begin[V];
+    }
+    if (total > kNaMaxItems)
+        return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_simulation_simulate_fused_noalloc: %d strand blocks (at most %d)",
+                    total, kNaMaxItems);
+    // staging launches: per object its records, bone matrices and work items
+    int bone_at = 0;
+    for (int i = 0; i < n; ++i) {
+        tfx_hair* h = hairs[i];
+        tfx_sdf* sdf = sdfs ? sdfs[i] : nullptr;
+        NaStage st;
+        std::memset(&st, 0, sizeof(st));
+        pack_dev_params(h, sdf, st.params);
+        st.hair = dev_hair(h, sdf);
+        st.hair.params = reinterpret_cast<const DevParams*>(b + kNaParamsOff) + i;
+        st.hair.bones = reinterpret_cast<const float*>(b + kNaBonesOff) + (size_t)16 * bone_at;
+        st.slot = i;
+        st.block_strands = cls_B[h->V];
+        for (int first = 0; first < h->bones_used || first == 0; first += kNaStageBones) {
+            st.bone_first = first;
+            st.nbones = std::min(kNaStageBones, h->bones_used - first);
+            std::memcpy(st.bones, h->bones_h.data() + (size_t)16 * first, (size_t)16 * st.nbones * sizeof(float));
+            st.item_first = item_first[i];
+            st.nitems = first == 0 ? nitems[i] : 0;
+            const int work = std::max(st.nitems, st.nbones * 16);
+            const int blocks = std::max(1, std::min((work + 255) / 256, 64));
+            hipLaunchKernelGGL(k_noalloc_stage, dim3(blocks), dim3(256), 0, s, b, st);
+            HIP_TRY(hipGetLastError());
+        }
+        bone_at += h->bones_used;
+    }
+    // the fused launches, the SDF collision inside them
+    const DevHair* dtab = reinterpret_cast<const DevHair*>(b + kNaHairsOff);
+    const tfx::WorkItem* ditems = reinterpret_cast<const tfx::WorkItem*>(b + kNaItemsOff);
+    for (int V = 4; V <= 64; V *= 2) {
+        const int cnt = cls_count[V];
+        if (!cnt) continue;
+        HIP_TRY(kl_launch_fused(V, cls_variant[V], dtab, ditems + cls_begin[V], cnt, std::min(cnt, cls_resident[V]), 0,
+                                s, nullptr, nullptr));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
     return TFX_OK;
