@@ -221,7 +221,7 @@ def fur_leg(torch, tfx, harness, stream, wl5, wls3, args):
     cells = int(np.prod(wl5.sdf_grid[2]))
     W5, K5, P5 = 2, args.fur_steps, args.fur_plain_steps
     legs = {}
-    timers = None
+    timers, vb5 = None, None
     for mode, k in (("graph", K5), ("fused", K5), ("plain", P5)):
         if k <= 0:
             continue
@@ -237,7 +237,9 @@ def fur_leg(torch, tfx, harness, stream, wl5, wls3, args):
             el5, ms5 = _timed_frames(torch, stream, d5, f5[W5:])
             if mode == "fused":
                 tfx.profiling_enable(False)
-                timers = tfx.profiling_read("FusedSubstep")
+                per_vb = {vb: tfx.profiling_read(f"FusedSubstep/V{V}/B{vb}") for vb in (512, 1024, 2048)}
+                vb5 = max(per_vb, key=lambda k: per_vb[k][0])
+                timers = (sum(t[0] for t in per_vb.values()), sum(t[1] for t in per_vb.values()))
         legs[mode] = {"value": n5 * subs * k / el5, "ms_per_step": el5 * 1e3 / k, "steps": k,
                       "latency_ms": {"p50": float(np.percentile(ms5, 50)), "p99": float(np.percentile(ms5, 99))}}
         _log(f"fur {mode} leg {el5 * 1e3 / k:.3f} ms/frame")
@@ -261,22 +263,28 @@ def fur_leg(torch, tfx, harness, stream, wl5, wls3, args):
             out["speedup_vs_plain"] = g["value"] / legs["plain"]["value"]
     if timers and timers[1]:
         avg_s = timers[0] / timers[1] / 1e3
-        out["roofline"] = {"bound": "hbm", "kernel": f"k_fused_substep<{V},*>", "avg_launch_us": avg_s * 1e6,
+        out["roofline"] = {"bound": "hbm", "kernel": fused_kernel_name(V, vb5, "fast" if tfx.get_arithmetic() ==
+                                                                  tfx.TFX_ARITH_FAST else "ieee"),
+                           "avg_launch_us": avg_s * 1e6,
                            "launches": timers[1], "algorithmic_bytes_per_launch": bytes_sub,
                            "achieved": bytes_sub / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": bytes_sub / avg_s / 1e9 / HBM_PEAK_GBS}
     if wls3 is not None and args.fur_served_seconds > 0:
         srv = tfx.HairServer(hbm_budget_bytes=200 << 30, max_instances=1000, substeps_per_frame=2,
                              substep_dt=float(np.float32(1.0 / synth.FRAME_RATE / 2)), max_inflight=2)
-        run = harness.ServedRun(srv, wls3, initial=args.served_initial, duration_s=args.fur_served_seconds,
+        initial, _ = served_initial(args)
+        run = harness.ServedRun(srv, wls3, initial=initial, duration_s=args.fur_served_seconds,
                                 inst_rate=20.0, lifetime_s=30.0, req_rate=5000.0, ks=(1, 2, 4), tenants=[wl5],
                                 tenant_hz=60.0)
-        _log(f"served window with the fur tenant, {args.fur_served_seconds:.0f} s")
+        _log(f"served window with the fur tenant, {args.fur_served_seconds:.0f} s, {initial} instances resident "
+             "at start")
         res = run.run(log=_log)
         srv.close()
         out["in_mix"] = {
-            "workload": "the config-4 request mix (as the served leg) with the fur resident as one more instance "
+            "workload": "the config-4 request mix (as the served leg, opened with "
+                        f"{initial} instances resident) with the fur resident as one more instance "
                         "(3 substeps per frame), its client requesting one frame every 1/60 s",
+            "resident_at_start": initial, "resident_at_end": res["resident_at_end"],
             "value": res["vertex_substeps"] / res["elapsed_s"], "unit": "vertex-substeps/s",
             "latency_ms": {"p50": res["latency_p50_ms"], "p99": res["latency_p99_ms"], "max": res["latency_max_ms"]},
             "tenant": res["tenants"][0], "passes": res["passes"],
@@ -300,13 +308,6 @@ def frame_sdf_bytes(wls):
     import numpy as np
     return sum(sdf_rebuild_bytes(int(np.prod(w.sdf_grid[2])), w.mesh.vertices(0.0).shape[0], w.mesh.num_triangles)
                for w in wls if w.mesh is not None)
-
-
-def block_vertices(V, variant):
-    """Guide vertices per strand block of a launch-shape variant: 512, 1024 or
-    2048, at most 128 strands."""
-    vb = 512 << variant
-    return 128 * V if vb // V > 128 else vb
 
 
 def fused_kernel_name(V, variant_vertices, arith):
@@ -348,7 +349,9 @@ def batched_leg(torch, tfx, harness, stream, wls, arith, W, K, plain_steps, tune
             for r in records or []:
                 kdrv.batch.set_launch_shape(r["vertices_per_strand"], r["variant"], r["blocks_per_cu"])
             lengths = sorted(class_bytes(wls))
-            labels = [f"FusedSubstep/V{V}" for V in lengths] + [
+            # fused launches are timed per strand length and strand-block size
+            fused_labels = {(V, vb): f"FusedSubstep/V{V}/B{vb}" for V in lengths for vb in (512, 1024, 2048)}
+            labels = list(fused_labels.values()) + [
                 "InitializeSignedDistanceField", "ConstructSignedDistanceField", "FinalizeSignedDistanceField",
                 "CollideHairVerticesWithSdf"]
             with torch.cuda.stream(stream):
@@ -372,15 +375,16 @@ def batched_leg(torch, tfx, harness, stream, wls, arith, W, K, plain_steps, tune
             out["graph_equals_direct"] = bool(same)
             del kdrv
             cb = class_bytes(wls)
-            shapes = {r["vertices_per_strand"]: r["variant"] for r in records or []}
             classes = []
             for V in lengths:
-                ms_tot, launches = timers[f"FusedSubstep/V{V}"]
-                if not launches:
+                by_vb = {vb: timers[lbl] for (v, vb), lbl in fused_labels.items() if v == V and timers[lbl][1]}
+                if not by_vb:
                     continue
+                ms_tot = sum(t[0] for t in by_vb.values())
+                launches = sum(t[1] for t in by_vb.values())
                 avg_s = ms_tot / launches / 1e3
-                vb = block_vertices(V, shapes[V]) if shapes.get(V, -1) >= 0 else None
-                classes.append({"vertices_per_strand": V, "kernel": fused_kernel_name(V, vb, arith) if vb else None,
+                vb = max(by_vb, key=lambda k: by_vb[k][0])      # the block size that took the time
+                classes.append({"vertices_per_strand": V, "kernel": fused_kernel_name(V, vb, arith),
                                 "algorithmic_bytes_per_launch": cb[V], "avg_launch_us": avg_s * 1e6,
                                 "launches": launches, "achieved": cb[V] / avg_s / 1e9,
                                 "frac": cb[V] / avg_s / 1e9 / HBM_PEAK_GBS,
@@ -436,6 +440,13 @@ def roofline_of(leg, traffic):
     }
 
 
+def served_initial(args):
+    """Instances resident when a served window opens: --served-initial, or the
+    Poisson steady state of the arrival process (20/s x 30 s mean lifetime)."""
+    steady = int(round(20.0 * 30.0))
+    return (args.served_initial if args.served_initial >= 0 else steady), steady
+
+
 def served_leg(tfx, harness, wls3, args, plain_value):
     """The served form in the fast arithmetic: a request window opened at the
     Poisson steady state of resident instances (arrival rate x mean lifetime),
@@ -443,8 +454,7 @@ def served_leg(tfx, harness, wls3, args, plain_value):
     arithmetics."""
     import numpy as np
     from tressfx_amd import synth
-    steady = int(round(20.0 * 30.0))
-    initial = args.served_initial if args.served_initial >= 0 else steady
+    initial, steady = served_initial(args)
     out = {}
     tfx.set_arithmetic(tfx.TFX_ARITH_FAST)
     try:

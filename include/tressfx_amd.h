@@ -204,7 +204,8 @@ const char* tfx_last_error(void);
  * TressFXSDFCollision.cpp:130-209).  While enabled, HIP events bracket every
  * kernel launch; tfx_profiling_read waits for and sums the launches recorded
  * under `label` or under any label extending it with '/' (a stage name,
- * "FusedSubstep" -- recorded per strand length as "FusedSubstep/V16" etc. --,
+ * "FusedSubstep" -- recorded per strand length and strand-block size as
+ * "FusedSubstep/V16/B512" etc., so "FusedSubstep/V16" reads one length --,
  * "InitializeSignedDistanceField", "ConstructSignedDistanceField",
  * "FinalizeSignedDistanceField", "CollideHairVerticesWithSdf") and forgets them. */
 int tfx_profiling_enable(int enable);
@@ -351,8 +352,9 @@ int tfx_simulation_simulate_fused_noalloc(tfx_simulation* sim, void* stream, tfx
 /* Launch shape of the fused kernel for strands of `vertices_per_strand`
  * vertices: `variant` 0, 1, 2 = strand blocks of 512, 1024, 2048 guide
  * vertices (at most 128 strands), -1 = chosen per call (fewest rounds of
- * resident blocks); `blocks_per_cu` caps the persistent grid per CU (0 = the
- * kernel's occupancy).  Results do not depend on the shape. */
+ * resident blocks); `blocks_per_cu` caps the persistent grid per CU for this
+ * strand length (0 = the kernel's occupancy).  Each strand length keeps its
+ * own shape.  Results do not depend on the shape. */
 int tfx_simulation_set_launch_shape(tfx_simulation* sim, int vertices_per_strand, int variant, int blocks_per_cu);
 /* As tfx_batch_set_sdf_deferral, for tfx_simulation_simulate_fused (values
  * TFX_SDF_DEFER_*, defined with the batched form below; default automatic). */

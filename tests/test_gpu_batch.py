@@ -282,6 +282,39 @@ def test_batch_membership_change(gpu):
          stream_of=lambda c: side.cuda_stream, graph_of=lambda c: True, batches=batches)
 
 
+def test_batch_launch_shapes_per_length_and_pass_timer_labels(gpu):
+    """Launch shapes pinned per strand length stay with their length (each
+    length's blocks-per-CU cap is its own), results equal the per-object
+    dispatch, and the pass timers record each fused launch under
+    "FusedSubstep/V<V>/B<block vertices>": reading a length or a length and
+    block size takes exactly its launches, "FusedSubstep" takes the rest."""
+    torch = gpu
+    print("[batch] pinned launch shapes per length and fused pass-timer labels", flush=True)
+    members = [Member(torch, i, *spec, copies=2) for i, spec in enumerate(MEMBERS)]
+    b = tfx.HairBatch()
+    for m in members:
+        b.add(m.hairs[1], m.sdfs[1])
+    b.set_launch_shape(16, 0, 4)      # 512-vertex blocks
+    b.set_launch_shape(32, 1, 1)      # 1024-vertex blocks, one per CU
+    b.set_launch_shape(64, 1, 3)      # 1024-vertex blocks
+    tfx.profiling_enable(True)
+    try:
+        tfx.profiling_read("FusedSubstep")
+        _run(torch, members, range(2), lambda f: 2, plain_copy=0, batch_copies=(1,), batches={1: b},
+             stream_of=lambda c: None, graph_of=lambda c: False)
+        torch.cuda.synchronize()
+        assert tfx.profiling_read("FusedSubstep/V16/B1024")[1] == 0
+        assert tfx.profiling_read("FusedSubstep/V16/B512")[1] == 4
+        assert tfx.profiling_read("FusedSubstep/V32")[1] == 4        # both 32-vertex members, one launch each
+        assert tfx.profiling_read("FusedSubstep/V32/B1024")[1] == 0  # already read
+        assert tfx.profiling_read("FusedSubstep/V64/B1024")[1] == 4
+        # the 4- and 8-vertex classes: one launch per substep each
+        ms, n = tfx.profiling_read("FusedSubstep")
+        assert n == 8 and ms > 0.0
+    finally:
+        tfx.profiling_enable(False)
+
+
 def test_batch_errors(gpu):
     b = tfx.HairBatch()
     st = tfx.SimulationSettings.defaults()

@@ -389,7 +389,7 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
             for (int V = 4; V <= 64; V *= 2) {
                 if (!b->vcount[V] || !b->vdefer[V]) continue;
                 const int grid = std::min(b->vcount[V], b->vresident[V]);
-                PassTimer tm(timers ? fused_label(V) : nullptr, s, true);
+                PassTimer tm(timers ? fused_label(V, kl_fused_block_vertices(V, b->vvariant[V])) : nullptr, s, true);
                 HIP_TRY(tm.launched(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V],
                                                     b->vcount[V], grid, 1, s, tm.start(), tm.stop())));
             }
@@ -399,7 +399,7 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
             if (!b->vcount[V]) continue;
             if (overlap && sub == 0 && b->vdefer[V]) continue;
             const int grid = std::min(b->vcount[V], b->vresident[V]);
-            PassTimer tm(timers ? fused_label(V) : nullptr, s, true);
+            PassTimer tm(timers ? fused_label(V, kl_fused_block_vertices(V, b->vvariant[V])) : nullptr, s, true);
             HIP_TRY(tm.launched(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V],
                                                 b->vcount[V], grid, b->vdefer[V], s, tm.start(), tm.stop())));
         }

@@ -32,7 +32,8 @@ extern "C" int tfx_profiling_enable(int enable) {
 extern "C" int tfx_profiling_read(const char* label, double* total_ms, int* launches) {
     if (!label || !total_ms || !launches) return fail(TFX_ERR_INVALID_ARGUMENT, "tfx_profiling_read: null argument");
     // `label` itself and every label that extends it with '/' (the fused
-    // launches are recorded per strand length, "FusedSubstep/V16", ...).
+    // launches are recorded per strand length and block size,
+    // "FusedSubstep/V16/B512", ...).
     std::vector<TimerRec> recs;
     {
         std::lock_guard<std::mutex> g(g_tmu);
@@ -471,8 +472,7 @@ extern "C" int tfx_simulation_set_launch_shape(tfx_simulation* sim, int V, int v
     CHECK_ARG(variant >= -1 && variant <= 2, "tfx_simulation_set_launch_shape: variant must be -1 (automatic), 0, 1 or 2");
     CHECK_ARG(blocks_per_cu >= 0 && blocks_per_cu <= 64, "tfx_simulation_set_launch_shape: blocks_per_cu out of range");
     sim->variant[V] = variant;
-    sim->grid_per_cu = blocks_per_cu;
-    sim->per_cu[V] = -1;
+    sim->per_cu[V] = blocks_per_cu;
     return TFX_OK;
 }
 
@@ -553,7 +553,7 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         const int cnt = vclass_count[V];
         if (!cnt) continue;
         const int grid = std::min(cnt, vclass_resident[V]);
-        PassTimer tm(fused_label(V), s, true);
+        PassTimer tm(fused_label(V, kl_fused_block_vertices(V, vclass_variant[V])), s, true);
         HIP_TRY(tm.launched(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid,
                                             defer[V], s, tm.start(), tm.stop())));
     }

@@ -171,16 +171,19 @@ struct PassTimer {
 }  // namespace tfx_timing
 using tfx_timing::PassTimer;
 
-// Pass-timer label of the fused launches of strand length V.
-inline const char* fused_label(int V) {
-    switch (V) {
-        case 4: return "FusedSubstep/V4";
-        case 8: return "FusedSubstep/V8";
-        case 16: return "FusedSubstep/V16";
-        case 32: return "FusedSubstep/V32";
-        case 64: return "FusedSubstep/V64";
-        default: return "FusedSubstep";
-    }
+// Pass-timer label of the fused launches of strand length V in strand blocks
+// of VB guide vertices: "FusedSubstep/V<V>/B<VB>" (the launch's kernel is
+// k_fused_substep<V, VB>).
+inline const char* fused_label(int V, int VB) {
+    static const char* const names[5][3] = {
+        {"FusedSubstep/V4/B512", "FusedSubstep/V4/B1024", "FusedSubstep/V4/B2048"},
+        {"FusedSubstep/V8/B512", "FusedSubstep/V8/B1024", "FusedSubstep/V8/B2048"},
+        {"FusedSubstep/V16/B512", "FusedSubstep/V16/B1024", "FusedSubstep/V16/B2048"},
+        {"FusedSubstep/V32/B512", "FusedSubstep/V32/B1024", "FusedSubstep/V32/B2048"},
+        {"FusedSubstep/V64/B512", "FusedSubstep/V64/B1024", "FusedSubstep/V64/B2048"}};
+    const int vi = V == 4 ? 0 : V == 8 ? 1 : V == 16 ? 2 : V == 32 ? 3 : V == 64 ? 4 : -1;
+    const int bi = VB == 512 ? 0 : VB == 1024 ? 1 : VB == 2048 ? 2 : -1;
+    return vi < 0 || bi < 0 ? "FusedSubstep" : names[vi][bi];
 }
 
 template <class T> inline hipError_t dalloc(T** p, size_t count) {
@@ -268,8 +271,7 @@ struct tfx_simulation {
     void* table_d = nullptr;   // DevHair[] followed by WorkItem[]
     size_t table_bytes = 0;
     int variant[65];           // launch-shape variant per V (-1: chosen per call from occupancy)
-    int grid_per_cu = 0;       // persistent blocks per CU (0: the kernel's occupancy)
-    int per_cu[65];            // per V, overrides grid_per_cu when >= 0 (set by the batch autotuner)
+    int per_cu[65];            // persistent blocks per CU per V (<= 0: the kernel's occupancy)
     int defer_sdf = TFX_SDF_DEFER_AUTO;  // batched form: SDF collision in the fused launch or a pass after it
 };
 
@@ -508,7 +510,7 @@ struct Shape { int variant; int resident; };
 inline Shape choose_shape(const tfx_simulation* sim, int V, tfx_hair* const* hairs, int n, int cus) {
     auto resident_of = [&](int variant) {
         const int occ = kl_fused_occupancy(V, variant);
-        const int cap = sim->per_cu[V] >= 0 ? sim->per_cu[V] : sim->grid_per_cu;
+        const int cap = sim->per_cu[V];
         const int per_cu = cap > 0 ? std::min(cap, std::max(occ, 1)) : std::max(occ, 1);
         return per_cu * cus;
     };

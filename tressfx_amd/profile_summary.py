@@ -3,9 +3,14 @@ a --kernel-trace --stats run and HBM bytes per dispatch from --pmc runs of
 FETCH_SIZE and WRITE_SIZE, calibrated against a device copy of known size.
 
     python -m tressfx_amd.profile_summary stats  KERNEL_STATS_CSV KERNEL_SUBSTRING
+    python -m tressfx_amd.profile_summary trace  KERNEL_TRACE_CSV KERNEL_SUBSTRING [LAST_N]
     python -m tressfx_amd.profile_summary traffic FETCH_CSV WRITE_CSV KERNEL_SUBSTRING COPY_BYTES OUT_JSON
     python -m tressfx_amd.profile_summary digest
     python -m tressfx_amd.profile_summary counters COUNTER_CSV KERNEL_SUBSTRING
+
+`trace` prints, per kernel whose name holds the substring, the duration
+statistics of all its dispatches and of its last LAST_N dispatches (the
+launches a bench leg timed, when that leg ran last).
 
 `counters` prints, for the dispatches of the kernel, the mean per dispatch of
 every counter of a --pmc run and the ratios read from SQ counters: VALU and
@@ -51,6 +56,28 @@ def kernel_stats(path, kernel):
             out.append({"name": r["Name"], "calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3,
                         "min_us": float(r["MinNs"]) / 1e3, "max_us": float(r["MaxNs"]) / 1e3,
                         "total_ms": float(r["TotalDurationNs"]) / 1e6})
+    return out
+
+
+def trace_stats(path, kernel, last_n=0):
+    """Per kernel: duration statistics (us) of every dispatch and of the last
+    `last_n` dispatches, in dispatch order."""
+    per = {}
+    for r in _rows(path):
+        if kernel in r["Kernel_Name"]:
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            per.setdefault(r["Kernel_Name"], []).append((int(r["Dispatch_Id"]), d))
+    out = []
+    for name, ds in per.items():
+        ds.sort()
+        durs = [d for _, d in ds]
+
+        def st(v):
+            return {"calls": len(v), "avg_us": statistics.mean(v), "min_us": min(v), "max_us": max(v)}
+        rec = {"name": name, "all": st(durs)}
+        if 0 < last_n <= len(durs):
+            rec[f"last_{last_n}"] = st(durs[-last_n:])
+        out.append(rec)
     return out
 
 
@@ -119,6 +146,9 @@ def counters(path, kernel):
 def main(argv):
     if argv[:1] == ["stats"]:
         for k in kernel_stats(argv[1], argv[2]):
+            print(json.dumps(k))
+    elif argv[:1] == ["trace"]:
+        for k in trace_stats(argv[1], argv[2], int(argv[3]) if len(argv) > 3 else 0):
             print(json.dumps(k))
     elif argv[:1] == ["traffic"]:
         r = traffic(argv[1], argv[2], argv[3], int(argv[4]))
