@@ -590,7 +590,9 @@ def main():
         for arith in ("fast", "ieee"):
             leg = batched_leg(torch, tfx, harness, stream, [wl2], arith, 10, args.config2_steps, args.plain_steps * 4,
                               tune=tune, label="config2")
-            leg["roofline"] = roofline_of(leg, None)
+            cls2 = leg.get("fused_classes") or [{}]
+            leg["roofline"] = roofline_of(leg, read_traffic(
+                os.path.join(REPO, "profiles", "fused_substep_config2_traffic.json"), cls2[0].get("kernel")))
             config2[arith] = leg
 
     plain_value = {"fast": head.get("plain", {}).get("value"),

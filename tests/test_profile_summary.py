@@ -62,6 +62,18 @@ def test_traffic_calibrated_by_the_copy(tmp_path):
     json.dumps(r)
 
 
+def test_traffic_last_n_takes_the_last_dispatches(tmp_path):
+    hdr = ["Kernel_Name", "Dispatch_Id", "Counter_Name", "Counter_Value"]
+    cal = [["__amd_rocclr_copyBuffer", 100 + i, "FETCH_SIZE", 1024.0] for i in range(3)]
+    rows = [["k_x", 9, "FETCH_SIZE", 30.0], ["k_x", 1, "FETCH_SIZE", 1000.0], ["k_x", 5, "FETCH_SIZE", 10.0]] + cal
+    p = _write(tmp_path / "f.csv", hdr, rows)
+    pw =_write(tmp_path / "w.csv", hdr, [[n, d, "WRITE_SIZE", v] for n, d, _, v in rows])
+    r = ps.traffic(p, pw, "k_x", 1 << 20, last_n=2)
+    assert r["FETCH_SIZE"]["kib_per_dispatch_mean"] == pytest.approx(20.0)     # dispatches 5 and 9
+    with pytest.raises(SystemExit):
+        ps.traffic(p, pw, "k_x", 1 << 20, last_n=4)
+
+
 def test_traffic_needs_a_calibration_copy(tmp_path):
     hdr = ["Kernel_Name", "Dispatch_Id", "Counter_Name", "Counter_Value"]
     pf = _write(tmp_path / "f.csv", hdr, [["k_x", 1, "FETCH_SIZE", 1.0]])

@@ -4,9 +4,9 @@ FETCH_SIZE and WRITE_SIZE, calibrated against a device copy of known size.
 
     python -m tressfx_amd.profile_summary stats  KERNEL_STATS_CSV KERNEL_SUBSTRING
     python -m tressfx_amd.profile_summary trace  KERNEL_TRACE_CSV KERNEL_SUBSTRING [LAST_N]
-    python -m tressfx_amd.profile_summary traffic FETCH_CSV WRITE_CSV KERNEL_SUBSTRING COPY_BYTES OUT_JSON
+    python -m tressfx_amd.profile_summary traffic FETCH_CSV WRITE_CSV KERNEL_SUBSTRING COPY_BYTES OUT_JSON [LAST_N]
     python -m tressfx_amd.profile_summary digest
-    python -m tressfx_amd.profile_summary counters COUNTER_CSV KERNEL_SUBSTRING
+    python -m tressfx_amd.profile_summary counters COUNTER_CSV KERNEL_SUBSTRING [LAST_N]
 
 `trace` prints, per kernel whose name holds the substring, the duration
 statistics of all its dispatches and of its last LAST_N dispatches (the
@@ -17,7 +17,8 @@ every counter of a --pmc run and the ratios read from SQ counters: VALU and
 LDS instructions per wave, VALU and LDS issue per wave-cycle, LDS bank
 conflicts per LDS issue cycle.
 
-Counter values are KiB per dispatch.  The copy kernel of the same run
+Counter values are KiB per dispatch (with LAST_N, the mean over the kernel's
+last LAST_N dispatches: a leg that ran last).  The copy kernel of the same run
 (__amd_rocclr_copyBuffer, COPY_BYTES read and written) gives the scale factor
 counter_bytes / true_bytes for each counter; a kernel's traffic is its counter
 bytes divided by that factor.  A traffic record carries the digest of the
@@ -82,21 +83,21 @@ def trace_stats(path, kernel, last_n=0):
 
 
 def _per_dispatch(path, counter):
-    """{kernel name: [counter value per dispatch]} (values summed over the
-    counter's instances within a dispatch)."""
+    """{kernel name: [counter value per dispatch]} in dispatch order (values
+    summed over the counter's instances within a dispatch)."""
     acc = {}
     for r in _rows(path):
         if r.get("Counter_Name") != counter:
             continue
-        key = (r["Kernel_Name"], r["Dispatch_Id"])
+        key = (r["Kernel_Name"], int(r["Dispatch_Id"]))
         acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
     per = {}
-    for (name, _), v in acc.items():
+    for (name, _), v in sorted(acc.items(), key=lambda kv: kv[0][1]):
         per.setdefault(name, []).append(v)
     return per
 
 
-def traffic(fetch_csv, write_csv, kernel, copy_bytes):
+def traffic(fetch_csv, write_csv, kernel, copy_bytes, last_n=0):
     res = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         per = _per_dispatch(path, counter)
@@ -109,6 +110,11 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
         ks = [v for n, vs in per.items() if kernel in n for v in vs]
         if not ks:
             raise SystemExit(f"{path}: no dispatches of {kernel}")
+        if last_n > 0:
+            names = {n for n in per if kernel in n}
+            if len(names) != 1 or last_n > len(ks):
+                raise SystemExit(f"{path}: LAST_N needs one kernel name with at least {last_n} dispatches")
+            ks = ks[-last_n:]
         res[counter] = {"kib_per_dispatch_mean": statistics.mean(ks), "dispatches": len(ks),
                         "copy_kib": statistics.mean(copies), "factor": factor,
                         "bytes_per_dispatch": statistics.mean(ks) * 1024.0 / factor}
@@ -119,11 +125,13 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
     return res
 
 
-def counters(path, kernel):
+def counters(path, kernel, last_n=0):
     names = sorted({r["Counter_Name"] for r in _rows(path)})
     out = {}
     for c in names:
         ks = [v for n, vs in _per_dispatch(path, c).items() if kernel in n for v in vs]
+        if last_n > 0:
+            ks = ks[-last_n:]
         if ks:
             out[c] = {"mean_per_dispatch": statistics.mean(ks), "dispatches": len(ks)}
     m = {c: v["mean_per_dispatch"] for c, v in out.items()}
@@ -151,12 +159,12 @@ def main(argv):
         for k in trace_stats(argv[1], argv[2], int(argv[3]) if len(argv) > 3 else 0):
             print(json.dumps(k))
     elif argv[:1] == ["traffic"]:
-        r = traffic(argv[1], argv[2], argv[3], int(argv[4]))
+        r = traffic(argv[1], argv[2], argv[3], int(argv[4]), int(argv[6]) if len(argv) > 6 else 0)
         with open(argv[5], "w") as f:
             json.dump(r, f, indent=1)
         print(json.dumps(r))
     elif argv[:1] == ["counters"]:
-        print(json.dumps(counters(argv[1], argv[2]), indent=1))
+        print(json.dumps(counters(argv[1], argv[2], int(argv[3]) if len(argv) > 3 else 0), indent=1))
     elif argv[:1] == ["digest"]:
         print(fused_source_digest())
     else:
