@@ -3,11 +3,13 @@ log-uniform in [500, 50,000] and 16, 32 or 64 vertices per strand (2.92 M
 strands, 76.4 M vertices), each with its own rigid trajectory, 64^3-128^3 SDF
 rebuilt per frame from a noisy icosphere, wind, 2 substeps.
 
-- Default arithmetic: the batch, every frame one replayed hipGraph, against
-  the reference's per-object dispatch (SDF Update, then per substep Simulate
-  over all objects and CollideWithHair per object) -- every instance's
-  positions, previous positions and tangents and every SDF grid bit for bit
-  after every frame.  The per-object dispatch of one instance per strand
+- Default arithmetic: the batch, every frame one replayed hipGraph, with the
+  automatic collision placement (inside the fused launches here) and with
+  every class's collision deferred to a collide pass (TFX_SDF_DEFER_ALWAYS,
+  the SDF build beside the first substep), against the reference's
+  per-object dispatch (SDF Update, then per substep Simulate over all objects
+  and CollideWithHair per object) -- every instance's positions, previous
+  positions and tangents and every SDF grid bit for bit after every frame.  The per-object dispatch of one instance per strand
   length and of the largest instance is checked substep by substep against
   the FP32 transcription (bit for bit, SDF grid included) and the FP64 one
   (1e-5 relative at every vertex), from the GPU's own state.
@@ -70,11 +72,16 @@ def test_config3_full_size_graph_batch_equals_per_object_and_oracles(gpu, worklo
           f"{[(i, wls[i].asset.num_guide_strands, wls[i].asset.num_vertices_per_strand) for i in checked]}",
           flush=True)
     bdrv = harness.BatchDriver(torch, wls, FRAMES)
+    # a second batch with every class's collision deferred to a collide pass
+    # (and the SDF build beside the first substep's launches)
+    ddrv = harness.BatchDriver(torch, wls, FRAMES, meshes=bdrv.meshes)
+    ddrv.batch.set_sdf_deferral(tfx.TFX_SDF_DEFER_ALWAYS)
     pdrv = harness.PlainMultiDriver(torch, wls, FRAMES, meshes=bdrv.meshes)
     stream = torch.cuda.Stream()
     worst = 0.0
     for f in FRAMES:
         bdrv.frame(f, stream.cuda_stream, use_graph=True)
+        ddrv.frame(f, stream.cuda_stream, use_graph=True)
         # the per-object dispatch, with the checked instances stepped against the oracles
         k = f - pdrv.first
         for sdf, d in zip(pdrv.sdfs, pdrv.descs[k]):
@@ -122,15 +129,20 @@ def test_config3_full_size_graph_batch_equals_per_object_and_oracles(gpu, worklo
                     worst = max(worst, check_fp64(got, ref, sens, f"frame {f} sub {s} instance {i}",
                                                   wl.asset.num_vertices_per_strand))
         torch.cuda.synchronize()
-        for i, (hb, hp) in enumerate(zip(bdrv.hairs, pdrv.hairs)):
-            keys = ("pos", "prev", "tangent")
-            a, b = hb.download_state(keys=keys), hp.download_state(keys=keys)
-            for key in keys:
-                mm = first_mismatch(a[key], b[key])
-                assert mm is None, f"frame {f} instance {i} (G={wls[i].asset.num_guide_strands}): {key} {mm}"
-        for i, (sb, sp) in enumerate(zip(bdrv.sdfs, pdrv.sdfs)):
-            assert np.array_equal(sb.download().view(np.uint32), sp.download().view(np.uint32)), f"frame {f} SDF {i}"
-        print(f"[config3] frame {f}: 256 instances bit-identical", flush=True)
+        keys = ("pos", "prev", "tangent")
+        for i, (hb, hd, hp) in enumerate(zip(bdrv.hairs, ddrv.hairs, pdrv.hairs)):
+            b = hp.download_state(keys=keys)
+            for name, h in (("batch", hb), ("deferred batch", hd)):
+                a = h.download_state(keys=keys)
+                for key in keys:
+                    mm = first_mismatch(a[key], b[key])
+                    assert mm is None, (f"frame {f} {name} instance {i} (G={wls[i].asset.num_guide_strands}): "
+                                        f"{key} {mm}")
+        for i, (sb, sd, sp) in enumerate(zip(bdrv.sdfs, ddrv.sdfs, pdrv.sdfs)):
+            want = sp.download().view(np.uint32)
+            assert np.array_equal(sb.download().view(np.uint32), want), f"frame {f} SDF {i}"
+            assert np.array_equal(sd.download().view(np.uint32), want), f"frame {f} deferred batch SDF {i}"
+        print(f"[config3] frame {f}: 256 instances bit-identical (batch, deferred batch, per-object)", flush=True)
     print(f"[config3] worst relative error vs FP64: {worst:.3e} (bound 1e-5 at every vertex)")
     moved = np.abs(pdrv.hairs[checked[-1]].download_state()["pos"][:, :3]
                    - wls[checked[-1]].asset.positions[:, :3]).max()

@@ -117,7 +117,13 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
     return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+#ifndef TFX_FAST_BUILD
 __device__ __forceinline__ f3 normalize3(f3 a) { return divs(a, sqrtf(dot3(a, a))); }
+#else
+// the hardware reciprocal square root: one transcendental instead of a square
+// root and a reciprocal
+__device__ __forceinline__ f3 normalize3(f3 a) { return mul(a, __builtin_amdgcn_rsqf(dot3(a, a))); }
+#endif
 // a / b, bit-identical to the IEEE quotient, for a divisor b shared by many
 // divisions, given y = recip_fixed(b): q = a*y, then twice r = a - b*q (exact
 // in an FMA) and q += r*y.  With y = 1/b correctly rounded, |a| >= 2^-96 (the
@@ -234,28 +240,24 @@ __device__ __forceinline__ float4 quat_two_vectors(f3 u, f3 v) {
 
 // MultQuaternionAndVector(QuatFromTwoUnitVectors(u, v), x): x rotated by the
 // rotation taking u onto v, the step of LocalShapeConstraints :808-814.  The
-// fast build rotates by the unnormalised quaternion q = (u x v, 1 + u.v)
-// directly, x + (2 / |q|^2) (q.w (q.xyz x x) + q.xyz x (q.xyz x x)), without
-// normalising it first; both branches of QuatFromTwoUnitVectors and the
-// near-zero case of NormalizeQuaternion :329-342 keep their meaning.
+// fast build applies that rotation by Rodrigues' formula with the axis and
+// sine folded into c = u x v and the cosine d = u.v:
+//   R x = d x + c x x + ((c.x) / (1 + d)) c
+// (no quaternion, one division); for antiparallel u and v (the branch of
+// QuatFromTwoUnitVectors :349-355) the half turn about an axis a
+// perpendicular to u, R x = 2 ((a.x) / (a.a)) a - x.
 __device__ __forceinline__ f3 rotate_from_to(f3 u, f3 v, f3 x) {
 #ifndef TFX_FAST_BUILD
     return qrot(quat_two_vectors(u, v), x);
 #else
-    float r = 1.0f + dot3(u, v);
-    f3 c;
+    const float d = dot3(u, v);
+    const float r = 1.0f + d;
     if (r < 1e-7f) {
-        r = 0.0f;
-        c = fabsf(u.x) > fabsf(u.z) ? f3{-u.y, u.x, 0.0f} : f3{0.0f, -u.z, u.y};
-    } else {
-        c = cross3(u, v);
+        const f3 a = fabsf(u.x) > fabsf(u.z) ? f3{-u.y, u.x, 0.0f} : f3{0.0f, -u.z, u.y};
+        return sub(mul(2.0f * dot3(a, x) / dot3(a, a), a), x);
     }
-    const float qq = dot3(c, c) + r * r;
-    if (qq < 1e-10f) return qrot(make_float4(c.x, c.y, c.z, 1.0f), x);
-    const float s = 2.0f / qq;
-    const f3 t = cross3(c, x);
-    const f3 ct = cross3(c, t);
-    return add(x, mul(s, add(mul(r, t), ct)));
+    const f3 c = cross3(u, v);
+    return add(add(mul(d, x), cross3(c, x)), mul(dot3(c, x) / r, c));
 #endif
 }
 
