@@ -179,6 +179,50 @@ extern "C" void tfx_simulation_settings_default(tfx_simulation_settings* s) {
 // ---------------------------------------------------------------------------
 // TressFXHairObject
 // ---------------------------------------------------------------------------
+// The fused kernel's processing order of the guide strands: by the Morton
+// code of each guide's root (vertex 0), 10 bits per axis over the roots'
+// bounding box, ties in index order.  A strand block then holds neighbouring
+// strands, whose SDF lookups share cells.  Empty when the order is the index
+// order already.
+static std::vector<int> guide_order(const float* positions, int G, int F, int V) {
+    std::vector<int> order;
+    if (G < 2) return order;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto root = [&](int g, int k) { return positions[4 * ((size_t)g * (F + 1) * V) + k]; };
+    for (int g = 0; g < G; ++g)
+        for (int k = 0; k < 3; ++k) {
+            const float x = root(g, k);
+            if (std::isfinite(x)) { lo[k] = std::min(lo[k], x); hi[k] = std::max(hi[k], x); }
+        }
+    auto spread = [](uint64_t x) {
+        x &= 0x3ff;
+        x = (x | (x << 16)) & 0x030000ffull;
+        x = (x | (x << 8)) & 0x0300f00full;
+        x = (x | (x << 4)) & 0x030c30c3ull;
+        x = (x | (x << 2)) & 0x09249249ull;
+        return x;
+    };
+    std::vector<uint64_t> key(G);
+    for (int g = 0; g < G; ++g) {
+        uint64_t code = 0;
+        for (int k = 0; k < 3; ++k) {
+            const float ext = hi[k] - lo[k];
+            const float x = root(g, k);
+            int q = 0;
+            if (ext > 0.0f && std::isfinite(x)) q = (int)std::min(1023.0f, std::max(0.0f, (x - lo[k]) / ext * 1023.0f));
+            code |= spread((uint64_t)q) << k;
+        }
+        key[g] = (code << 32) | (uint64_t)g;   // unique keys: ties keep index order
+    }
+    std::vector<int> idx(G);
+    for (int g = 0; g < G; ++g) idx[g] = g;
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key[a] < key[b]; });
+    bool identity = true;
+    for (int g = 0; g < G && identity; ++g) identity = idx[g] == g;
+    if (!identity) order.swap(idx);
+    return order;
+}
+
 extern "C" int tfx_hair_create(const tfx_asset_desc* a, tfx_hair** out) {
     CHECK_ARG(a && out, "tfx_hair_create: null argument");
     *out = nullptr;
@@ -250,6 +294,19 @@ extern "C" int tfx_hair_create(const tfx_asset_desc* a, tfx_hair** out) {
     HIP_TRY(hipMemset(h->sld, 0, (size_t)S * sizeof(tfx_strand_level_data)));
     HIP_TRY(hipMemcpy(h->bones_d, h->bones_h.data(), h->bones_h.size() * sizeof(float), hipMemcpyHostToDevice));
     h->bones_dirty = 0;
+    {
+        const std::vector<int> order = guide_order(a->positions, h->G, h->F, V);
+        if (!order.empty()) {
+            hipError_t oe = dalloc(&h->order, order.size());
+            if (oe == hipSuccess) oe = hipMemcpy(h->order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice);
+            if (oe != hipSuccess) {
+                hair_free(h);
+                delete h;
+                return fail(oe == hipErrorOutOfMemory ? TFX_ERR_OUT_OF_MEMORY : TFX_ERR_HIP, "tfx_hair_create: %s",
+                            hipGetErrorString(oe));
+            }
+        }
+    }
     compute_tangent_w(h, a->tangents);
     // The matrices the kernels can read (guide strands' skinning records).
     h->bones_used = skin_bones_used(a->bone_skinning_data, S, h->F + 1);

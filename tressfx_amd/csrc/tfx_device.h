@@ -442,6 +442,12 @@ struct DevHair {
     const float* sdf;            // float cells of the hair's SDF (fused path)
     const uint8_t* sdf_bricks;   // its dilated brick occupancy
     const uint32_t* sdf_brick_bits;  // the same, one bit per brick
+    // The fused kernel's processing order of the guide strands: work item
+    // (hair, g0) takes strands order[g0], order[g0 + 1], ... (nullptr: g0,
+    // g0 + 1, ...) when the launch collides with the SDF.  Strands are
+    // independent within a substep, so the order changes no result; a
+    // spatially coherent one makes the strands of a block share SDF cells.
+    const int* order;
     int G, F, V, pad;
 };
 

@@ -61,6 +61,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     __shared__ float4 sV01[B][3];      // P1->P2 prev(v0), prev(v1), prevprev(v1)
     __shared__ float4 sOff[B][kMaxStagedFollow];  // P0->P6b follow root offsets (F <= kMaxStagedFollow)
     __shared__ uint32_t sBits[kMaxBrickWords];    // SDF brick occupancy, one bit per brick
+    __shared__ int sIdx[B];            // P1->P6b the item's guide strands (the object's processing order)
     __shared__ int sQueueCount;
 
     const int t = threadIdx.x;
@@ -71,9 +72,15 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     // P0 of a work item: the blended bone and skinning quaternion of each of
     // its strands, threads first, first + count, ... taking one strand each.
     // `lean`: with the item's vertex registers live (during P4).
+    // The object's processing order, used when the launch makes its SDF
+    // lookups (their cells are what neighbouring strands share); without
+    // them index order keeps each block's streams contiguous.
+    auto order_of = [&](const DevHair& hh) -> const int* { return hh.sdf != nullptr && !defer_sdf ? hh.order : nullptr; };
     auto blend_strands = [&](const DevHair& hh, int gg0, int nnb, int first, int count, auto lean) {
+        const int* ord = order_of(hh);
         for (int s = first; s < nnb; s += count) {
-            const size_t strand = (size_t)(gg0 + s) * (hh.F + 1);
+            const int gs = ord ? ldg(&ord[gg0 + s]) : gg0 + s;
+            const size_t strand = (size_t)gs * (hh.F + 1);
             BlendedBone bb;
             if constexpr (decltype(lean)::value) blend_bones_lean(ldg(&hh.skin[strand]), hh.bones, bb);
             else blend_bones(ldg(&hh.skin[strand]), hh.bones, bb);
@@ -109,17 +116,19 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             for (int k = t; k < brick_words; k += kFusedBlock) sBits[k] = ldg(&h.sdf_brick_bits[k]);
 
         // ---- P0: blended bone per strand ---------------------------------
+        const int* ord = order_of(h);
+        for (int s = t; s < nb; s += kFusedBlock) sIdx[s] = ord ? ldg(&ord[g0 + s]) : g0 + s;
         if (!blended) blend_strands(h, g0, nb, tt, kFusedBlock, std::false_type{});
         if (F > 0 && F <= kMaxStagedFollow)
             for (int s = tt; s < nb; s += kFusedBlock) {
-                const size_t strand = (size_t)(g0 + s) * (F + 1);
+                const size_t strand = (size_t)(ord ? ldg(&ord[g0 + s]) : g0 + s) * (F + 1);
                 for (int j = 0; j < F; ++j) sOff[s][j] = ldg(&h.follow_off[strand + 1 + j]);
             }
         __syncthreads();
 
         // ---- P1: integration and global shape ------------------------------
         // Every global load of the phase is issued before any of its results
-        // is used (lanes past the block's last strand read strand g0's
+        // is used (lanes past the block's last strand read the first strand's
         // vertices and discard them).
         float4 prevReg[NPT];
         float restLen[NPT];
@@ -128,7 +137,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         for (int k = 0; k < NPT; ++k) {
             const int lv = k * kFusedBlock + t;
             const int ls = lv / V, v = lv % V;
-            const size_t i = ((size_t)(g0 + (ls < nb ? ls : 0)) * (F + 1)) * V + v;
+            const size_t i = ((size_t)sIdx[ls < nb ? ls : 0] * (F + 1)) * V + v;
             restIn[k] = ldg(&h.rest[i]);
             posIn[k] = ldg(&h.pos[i]);
             prevIn[k] = ldg(&h.prev[i]);
@@ -144,7 +153,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             const int ls = lv / V, v = lv % V;
             prevReg[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             const bool valid = ls < nb;
-            const size_t i = ((size_t)(g0 + (valid ? ls : 0)) * (F + 1)) * V + v;
+            const size_t i = ((size_t)sIdx[valid ? ls : 0] * (F + 1)) * V + v;
             float4 initialPos = valid ? restIn[k] : make_float4(0.f, 0.f, 0.f, 0.f);
             // Bind pose of LocalShapeConstraints, MultQuaternionAndVector(boneQuat,
             // rest) (:799-801); a strand's vertices sit in consecutive lanes of one
@@ -314,7 +323,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 const int ls = lv / V, v = lv % V;
                 force[k] = f3{0.f, 0.f, 0.f};
                 if (ls < nb && v >= 2 && v < V - 1) {
-                    const int strand = (g0 + ls) * (F + 1);
+                    const int strand = sIdx[ls] * (F + 1);
                     float a = (float)(uint32_t)(strand % 20) / 20.0f;
                     f3 wv = add(add(add(mul(a, xyz(ld4(P.wind0))), mul(1.0f - a, xyz(ld4(P.wind1)))), mul(a, xyz(ld4(P.wind2)))),
                                 mul(1.0f - a, xyz(ld4(P.wind3))));
@@ -439,7 +448,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 const int lv = k * kFusedBlock + t;
                 const int ls = lv / V, v = lv % V;
                 if (ls < nb) {
-                    const size_t i = ((size_t)(g0 + ls) * (F + 1)) * V + v;
+                    const size_t i = ((size_t)sIdx[ls] * (F + 1)) * V + v;
                     const int base = ls * STRIDE;
                     const int ti = (v == V - 1) ? v - 1 : v;
                     const f3 tan = normalize3(sub(xyz(sPos[base + ti + 1]), xyz(sPos[base + ti])));
@@ -485,7 +494,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 const int lv = idx - (j - 1) * per;
                 const int ls = lv / V;
                 v = lv % V;
-                const size_t strand = (size_t)(g0 + ls) * (F + 1);
+                const size_t strand = (size_t)sIdx[ls] * (F + 1);
                 fi = (strand + j) * V + v;
                 const float4 sp = sPos[ls * STRIDE + v];
                 ft = sVB[ls * STRIDE + v];
@@ -516,7 +525,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 const int j = idx / per + 1;
                 const int lv = idx - (j - 1) * per;
                 const int ls = lv / V, v = lv % V;
-                fi = ((size_t)(g0 + ls) * (F + 1) + j) * V + v;
+                fi = ((size_t)sIdx[ls] * (F + 1) + j) * V + v;
                 ft = sVB[ls * STRIDE + v];
             };
             for (int q0 = t; q0 < nq; q0 += 4 * kFusedBlock) {

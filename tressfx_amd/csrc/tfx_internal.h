@@ -253,6 +253,7 @@ struct tfx_hair {
     tfx_bone_skinning_data* skin = nullptr;
     float* bones_d = nullptr;
     DevParams* params_d = nullptr;
+    int* order = nullptr;                 // fused kernel's guide-strand order (nullptr: index order)
 
     tfx_simulation_params params{};       // m_SimCB (single block: kernel arguments are copied at launch)
     std::vector<float> bones_h;           // 512 x 16
@@ -381,7 +382,7 @@ inline void set_identity_bones(std::vector<float>& b) {
 inline void hair_free(tfx_hair* h) {
     (void)hipFree(h->pos); (void)hipFree(h->prev); (void)hipFree(h->prevprev); (void)hipFree(h->tangent);
     (void)hipFree(h->sld); (void)hipFree(h->rest); (void)hipFree(h->restlen); (void)hipFree(h->follow_off);
-    (void)hipFree(h->skin); (void)hipFree(h->bones_d); (void)hipFree(h->params_d);
+    (void)hipFree(h->skin); (void)hipFree(h->bones_d); (void)hipFree(h->params_d); (void)hipFree(h->order);
     if (h->handoff) (void)hipEventDestroy(h->handoff);
 }
 
@@ -477,6 +478,7 @@ inline DevHair dev_hair(const tfx_hair* h, const tfx_sdf* sdf) {
     d.sdf = sdf ? reinterpret_cast<const float*>(sdf->cells) : nullptr;
     d.sdf_bricks = sdf ? sdf->bricks : nullptr;
     d.sdf_brick_bits = sdf ? sdf->brick_bits : nullptr;
+    d.order = h->order;
     d.G = h->G; d.F = h->F; d.V = h->V; d.pad = 0;
     return d;
 }
