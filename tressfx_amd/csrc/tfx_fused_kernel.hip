@@ -32,6 +32,14 @@ constexpr int kFusedBlock = 256;
 constexpr int kMaxStagedFollow = 4;  // follow strands per guide whose offsets are staged in LDS
 constexpr int kMaxBrickWords = 1024; // SDF grids of up to 32768 bricks keep their occupancy in LDS
 
+// Orders a wave's LDS accesses before and after this point (LDS executes a
+// wave's accesses in order; the fences keep the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int V, int VB>
 struct FusedShape {
     static constexpr int B = VB / V;                 // guide strands per block
@@ -315,6 +323,10 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         __syncthreads();
 
         // ---- P5: wind, length constraints, capsules ------------------------
+        // In the vertex-parallel mapping a strand's vertices lie in
+        // consecutive lanes of one wave (64 % V == 0), and P5 reads only
+        // vertices of the strand it writes, so its steps (and P6a's reads of
+        // them) are ordered within each wave instead of across the block.
         if (P.wind_on) {
             f3 force[NPT];
 #pragma unroll
@@ -332,7 +344,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     force[k] = f3{-c.x, -c.y, -c.z};
                 }
             }
-            __syncthreads();
+            wave_sync();
 #pragma unroll
             for (int k = 0; k < NPT; ++k) {
                 const int lv = k * kFusedBlock + t;
@@ -343,7 +355,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     sp.x += f.x; sp.y += f.y; sp.z += f.z;
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
         for (int it = 0; it < P.length_iters; ++it) {
 #pragma unroll
@@ -356,7 +368,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     sPos[ls * STRIDE + v] = a; sPos[ls * STRIDE + v + 1] = b;
                 }
             }
-            __syncthreads();
+            wave_sync();
 #pragma unroll
             for (int k = 0; k < NPT; ++k) {
                 const int lv = k * kFusedBlock + t;
@@ -367,7 +379,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     sPos[ls * STRIDE + v] = a; sPos[ls * STRIDE + v + 1] = b;
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
         uint32_t hitMask = 0;
         if (P.num_capsules > 0) {
@@ -387,7 +399,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     sPos[ls * STRIDE + v] = cur;
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
 
         // ---- P6a: guide tangent, clamp, SDF collision, stores ---------------
