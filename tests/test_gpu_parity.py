@@ -188,15 +188,23 @@ def test_sdf_reference_grid_sizing(gpu):
     assert bits_equal(g, ref)
 
 
-@pytest.mark.parametrize("deferral", ["auto", "never"])
-def test_collide_and_fused_vs_oracle(gpu, deferral):
+@pytest.mark.parametrize("deferral,follow_w", [("auto", "guide"), ("never", "guide"), ("auto", "own")])
+def test_collide_and_fused_vs_oracle(gpu, deferral, follow_w):
     """Plain Simulate + CollideWithHair and the fused substep, stepwise against
     the FP32 oracle (bit-exact) and the FP64 oracle (1e-5), with SDF, wind and
     follow strands; the fused substep's SDF collision as a collide pass after
     its launch (auto: this object fits in one round of resident blocks) or
-    inside it (never)."""
+    inside it (never).  follow_w: the follow vertices carry their guide
+    vertex's w, as GenerateFollowHairs makes them (the fused kernel writes
+    them as whole records), or a w of their own, which UpdateFollowHairVertices
+    keeps (it writes x, y and z)."""
     torch = gpu
     wl, dm = _mesh_setup(torch)
+    if follow_w == "own":
+        a = wl.asset
+        G, F, V = a.num_guide_strands, a.num_follow_strands_per_guide, a.num_vertices_per_strand
+        pos = a.positions.reshape(G, F + 1, V, 4)
+        pos[:, 1:, 2:, 3] = 0.5          # movable, but not the guide's 1.0
     origin, cell, dims = wl.sdf_grid
     system = tfx.TressFXSDFCollisionSystem()
     sim = tfx.TressFXSimulation()

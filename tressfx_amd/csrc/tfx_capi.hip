@@ -308,6 +308,7 @@ extern "C" int tfx_hair_create(const tfx_asset_desc* a, tfx_hair** out) {
         }
     }
     compute_tangent_w(h, a->tangents);
+    compute_follow_w(h, a->positions);
     // The matrices the kernels can read (guide strands' skinning records).
     h->bones_used = skin_bones_used(a->bone_skinning_data, S, h->F + 1);
     // Constant block before the first UpdateSimulationParameters: zero, with
@@ -455,6 +456,7 @@ extern "C" int tfx_hair_upload_state(tfx_hair* h, void* stream, const float* pos
     if (sld) HIP_TRY(hipMemcpyAsync(h->sld, sld, (size_t)h->S * sizeof(tfx_strand_level_data), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (tan) compute_tangent_w(h, tan);
+    if (pos) compute_follow_w(h, pos);
     return TFX_OK;
 }
 

@@ -448,7 +448,12 @@ struct DevHair {
     // independent within a substep, so the order changes no result; a
     // spatially coherent one makes the strands of a block share SDF cells.
     const int* order;
-    int G, F, V, pad;
+    int G, F, V;
+    // 1 when every follow vertex's w equals its guide vertex's w (as
+    // GenerateFollowHairs makes them, TressFXAsset.cpp:234): the fused
+    // kernel then writes a follow position as a whole record with the
+    // guide's w instead of x, y and z one by one.
+    int follow_w_guide;
 };
 
 // One entry of the fused kernel's work list: a block of guide strands of one

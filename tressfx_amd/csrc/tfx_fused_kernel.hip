@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             const float tip = P.tip_sep;
             const bool staged = F <= kMaxStagedFollow;
             uint16_t* queue = reinterpret_cast<uint16_t*>(sNB);   // 4 * B * STRIDE >= F * VB entries when staged
-            auto follow_vertex = [&](int idx, size_t& fi, f3& fp, float4& ft, int& v) {
+            auto follow_vertex = [&](int idx, size_t& fi, f3& fp, float4& ft, int& v, float& gw) {
                 const int j = idx / per + 1;
                 const int lv = idx - (j - 1) * per;
                 const int ls = lv / V;
@@ -509,6 +509,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 const size_t strand = (size_t)sIdx[ls] * (F + 1);
                 fi = (strand + j) * V + v;
                 const float4 sp = sPos[ls * STRIDE + v];
+                gw = sp.w;
                 ft = sVB[ls * STRIDE + v];
                 const float factor = tip * ((float)v / (float)V) + 1.0f;
                 const f3 off = xyz(staged ? sOff[ls][j - 1] : ldg(&h.follow_off[strand + j]));
@@ -520,11 +521,13 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     f3 fp;
                     float4 ft;
                     int v;
-                    follow_vertex(idx, fi, fp, ft, v);
+                    float gw;
+                    follow_vertex(idx, fi, fp, ft, v, gw);
                     if (has_sdf && v >= 2 && maybe_near(fp)) {
                         queue[atomicAdd(&sQueueCount, 1)] = (uint16_t)idx;
                     } else {
-                        store_xyz(&h.pos[fi], fp);
+                        if (h.follow_w_guide) stg(&h.pos[fi], make_float4(fp.x, fp.y, fp.z, gw));
+                        else store_xyz(&h.pos[fi], fp);
                         stg(&h.tangent[fi], ft);
                     }
                 }
@@ -555,8 +558,9 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                         int v;
                         size_t fi;
                         float4 ft;
+                        float gw;
                         qi[u] = staged ? (int)queue[q] : q;
-                        follow_vertex(qi[u], fi, fp[u], ft, v);
+                        follow_vertex(qi[u], fi, fp[u], ft, v, gw);
                         act[u] = has_sdf && v >= 2 && (staged || maybe_near(fp[u]));
                     }
                 }

@@ -261,6 +261,7 @@ struct tfx_hair {
     int frame = 0;                        // m_SimulationFrame
     int bones_used = 1;                   // 1 + the largest bone index the skinning data references
     int tangent_w_uniform = 1;
+    int follow_w_guide = 0;               // every follow vertex's w equals its guide vertex's w
     float tangent_w = 0.0f;
     Staging staging;
     hipEvent_t handoff = nullptr;         // sim <-> rendering stream barrier (created on first use)
@@ -386,6 +387,21 @@ inline void hair_free(tfx_hair* h) {
     if (h->handoff) (void)hipEventDestroy(h->handoff);
 }
 
+// Whether every follow vertex of `positions` (the object's layout) carries
+// its guide vertex's w, bit for bit.
+inline void compute_follow_w(tfx_hair* h, const float* positions) {
+    h->follow_w_guide = 0;
+    if (h->F == 0 || !positions) return;
+    for (int g = 0; g < h->G; ++g) {
+        const size_t base = (size_t)g * (h->F + 1) * h->V;
+        for (int j = 1; j <= h->F; ++j)
+            for (int v = 0; v < h->V; ++v)
+                if (std::memcmp(&positions[4 * (base + (size_t)j * h->V + v) + 3], &positions[4 * (base + v) + 3], 4))
+                    return;
+    }
+    h->follow_w_guide = 1;
+}
+
 inline void compute_tangent_w(tfx_hair* h, const float* tangents) {
     h->tangent_w_uniform = 1;
     h->tangent_w = 0.0f;
@@ -479,7 +495,7 @@ inline DevHair dev_hair(const tfx_hair* h, const tfx_sdf* sdf) {
     d.sdf_bricks = sdf ? sdf->bricks : nullptr;
     d.sdf_brick_bits = sdf ? sdf->brick_bits : nullptr;
     d.order = h->order;
-    d.G = h->G; d.F = h->F; d.V = h->V; d.pad = 0;
+    d.G = h->G; d.F = h->F; d.V = h->V; d.follow_w_guide = h->follow_w_guide;
     return d;
 }
 
