@@ -501,9 +501,8 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             const float tip = P.tip_sep;
             const bool staged = F <= kMaxStagedFollow;
             uint16_t* queue = reinterpret_cast<uint16_t*>(sNB);   // 4 * B * STRIDE >= F * VB entries when staged
-            auto follow_vertex = [&](int idx, size_t& fi, f3& fp, float4& ft, int& v, float& gw) {
-                const int j = idx / per + 1;
-                const int lv = idx - (j - 1) * per;
+            // follow strand j (1..F) of the item's guide vertex lv
+            auto follow_vertex_at = [&](int j, int lv, size_t& fi, f3& fp, float4& ft, int& v, float& gw) {
                 const int ls = lv / V;
                 v = lv % V;
                 const size_t strand = (size_t)sIdx[ls] * (F + 1);
@@ -515,20 +514,28 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 const f3 off = xyz(staged ? sOff[ls][j - 1] : ldg(&h.follow_off[strand + j]));
                 fp = add(xyz(sp), mul(factor, off));
             };
+            auto follow_vertex = [&](int idx, size_t& fi, f3& fp, float4& ft, int& v, float& gw) {
+                const int j = idx / per + 1;
+                follow_vertex_at(j, idx - (j - 1) * per, fi, fp, ft, v, gw);
+            };
             if (staged) {
-                for (int idx = t; idx < nf; idx += kFusedBlock) {
-                    size_t fi;
-                    f3 fp;
-                    float4 ft;
-                    int v;
-                    float gw;
-                    follow_vertex(idx, fi, fp, ft, v, gw);
-                    if (has_sdf && v >= 2 && maybe_near(fp)) {
-                        queue[atomicAdd(&sQueueCount, 1)] = (uint16_t)idx;
-                    } else {
-                        if (h.follow_w_guide) stg(&h.pos[fi], make_float4(fp.x, fp.y, fp.z, gw));
-                        else store_xyz(&h.pos[fi], fp);
-                        stg(&h.tangent[fi], ft);
+                // strand by strand, so that no lane divides by the item's size
+                for (int j = 1; j <= F; ++j) {
+                    for (int lv = t; lv < per; lv += kFusedBlock) {
+                        const int idx = (j - 1) * per + lv;
+                        size_t fi;
+                        f3 fp;
+                        float4 ft;
+                        int v;
+                        float gw;
+                        follow_vertex_at(j, lv, fi, fp, ft, v, gw);
+                        if (has_sdf && v >= 2 && maybe_near(fp)) {
+                            queue[atomicAdd(&sQueueCount, 1)] = (uint16_t)idx;
+                        } else {
+                            if (h.follow_w_guide) stg(&h.pos[fi], make_float4(fp.x, fp.y, fp.z, gw));
+                            else store_xyz(&h.pos[fi], fp);
+                            stg(&h.tangent[fi], ft);
+                        }
                     }
                 }
                 __syncthreads();
