@@ -251,15 +251,17 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
 
         // ---- P4: local shape constraints (Gauss–Seidel sweeps) -------------
         // Step i of a pass reads vertex i+1 from LDS (vertices i-1 and i come
-        // from its own previous step) and stores vertex i, whose value is then
-        // final for the pass (the last step also stores vertex V-1).  So up to
-        // kLscPipe passes run together, pass k on its own lane kLscLag = 2
-        // steps behind pass k-1: pass k's step i runs one step after pass k-1
-        // stored vertex i+1, and pass k-1 never reads a vertex that pass k
-        // has stored.  Every step reads exactly the values the sequential
-        // order gives it.  The lanes of one strand are adjacent in one wave;
-        // the wave runs the steps in lock step and orders its LDS accesses by
-        // a wave-scope fence per step.
+        // from its own previous step), stores vertex i, which is then final
+        // for the pass, and stores vertex i+1 as the step left it (step i+1
+        // replaces it; after the last step it is vertex V-1's final value).
+        // So up to kLscPipe passes run together, pass k on its own lane
+        // kLscLag = 2 steps behind pass k-1: pass k reads vertex i+1 at its
+        // step i, one step after pass k-1 stored it final at its step i+1,
+        // from when on pass k-1 stores only vertices above i+1; and pass k-1
+        // never reads a vertex that pass k has stored.  Every step reads
+        // exactly the values the sequential order gives it.  The lanes of one
+        // strand are adjacent in one wave; the wave runs the steps in lock
+        // step and orders its LDS accesses by a wave-scope fence per step.
         // The waves the sweep leaves idle compute P0 of the block's next work
         // item meanwhile (sMat and sSkinQ are dead from P4 on: P1 last read
         // them, and P2/P3 their VSP reuse).
@@ -304,11 +306,22 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                             const f3 nLastBind{vbw.w, nyz.x, nyz.y};
                             f3 lastVec = sub(xyz(p), xyz(pm));
                             f3 org = add(rotate_from_to(nLastBind, normalize3(lastVec), vecBindPose), xyz(p));
+#ifndef TFX_FAST_BUILD
                             f3 del = mul(stiff, sub(org, xyz(pp)));
                             if (movable(p)) { p.x -= del.x; p.y -= del.y; p.z -= del.z; }
                             if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
+#else
+                            // the stiffness of an immovable vertex is zero
+                            const f3 dif = sub(org, xyz(pp));
+                            const float sp0 = movable(p) ? stiff : 0.0f, sp1 = movable(pp) ? stiff : 0.0f;
+                            p.x -= sp0 * dif.x; p.y -= sp0 * dif.y; p.z -= sp0 * dif.z;
+                            pp.x += sp1 * dif.x; pp.y += sp1 * dif.y; pp.z += sp1 * dif.z;
+#endif
+                            // vertex lv is final for this pass; lv+1 is final
+                            // once the next step has moved it (or, after the
+                            // last step, now: vertex V-1)
                             sp[lv] = p;
-                            if (lv == V - 2) sp[V - 1] = pp;
+                            sp[lv + 1] = pp;
                             pm = p; p = pp;
                         }
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
