@@ -56,6 +56,12 @@ __device__ __forceinline__ T ldg(const T* p) {
         T out;
         __builtin_memcpy(&out, &v, 16);
         return out;
+    } else if constexpr (sizeof(T) == 8 && alignof(T) >= 8) {
+        typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+        const u2 v = *(const TFX_ADDRSPACE(1) u2*)p;
+        T out;
+        __builtin_memcpy(&out, &v, 8);
+        return out;
     } else if constexpr (sizeof(T) > 16 && sizeof(T) % 4 == 0) {
         // larger records (the 32-byte skinning record) word by word
         const TFX_ADDRSPACE(1) unsigned int* w = (const TFX_ADDRSPACE(1) unsigned int*)p;
@@ -260,6 +266,36 @@ __device__ __forceinline__ f3 rotate_from_to(f3 u, f3 v, f3 x) {
     return add(add(mul(d, x), cross3(c, x)), mul(dot3(c, x) / r, c));
 #endif
 }
+
+#ifdef TFX_FAST_BUILD
+// The fast build's goal position of one LocalShapeConstraints step
+// (:799-829): p + R x, R the rotation taking the unit bind direction u onto
+// the direction of lastVec = p - pm, x the bind-pose edge.  With v = lastVec
+// / |lastVec| and s = u + v,
+//   R x = x - ((s.x) / (1 + u.v)) s + 2 (u.x) v,
+// evaluated from lastVec's three dot products and one reciprocal square root,
+// so that the step's dependent chain is short; antiparallel u and v take the
+// half turn of rotate_from_to.
+__device__ __forceinline__ f3 lsc_target(f3 u, f3 x, f3 pm, f3 p) {
+    const f3 lv = sub(p, pm);
+    const float ll = dot3(lv, lv), ul = dot3(u, lv), xl = dot3(x, lv), ux = dot3(u, x);
+    const float rs = __builtin_amdgcn_rsqf(ll);
+    const float r = 1.0f + ul * rs;
+    const f3 v = mul(lv, rs);
+    const float k = (ux + xl * rs) / r;
+    const float w = 2.0f * ux;
+    const f3 base = add(p, x);
+    f3 out{base.x + w * v.x - k * (u.x + v.x), base.y + w * v.y - k * (u.y + v.y), base.z + w * v.z - k * (u.z + v.z)};
+    const bool anti = r < 1e-7f;
+    if (__builtin_expect(__any(anti), 0)) {
+        if (anti) {
+            const f3 a = fabsf(u.x) > fabsf(u.z) ? f3{-u.y, u.x, 0.0f} : f3{0.0f, -u.z, u.y};
+            out = add(sub(mul(2.0f * dot3(a, x) / dot3(a, a), a), x), p);
+        }
+    }
+    return out;
+}
+#endif
 
 // Blended bone of one strand: ApplyVertexBoneSkinning's matrix path
 // (:549-567) — 3x3 rotation/scale part plus translation row.

@@ -139,7 +139,6 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         // is used (lanes past the block's last strand read the first strand's
         // vertices and discard them).
         float4 prevReg[NPT];
-        float restLen[NPT];
         float4 restIn[NPT], posIn[NPT], prevIn[NPT];
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
@@ -149,7 +148,23 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             restIn[k] = ldg(&h.rest[i]);
             posIn[k] = ldg(&h.pos[i]);
             prevIn[k] = ldg(&h.prev[i]);
-            restLen[k] = ldg(&h.restlen[i]);
+        }
+        // The length constraints take one lane per vertex pair (P5): pair
+        // slot m of this thread starts at block vertex pairLv[m] (even phase;
+        // the odd phase's pair starts one vertex later).  Lanes 0-31 of a wave
+        // take the pairs of the 64 vertices the wave holds for k = 2m, lanes
+        // 32-63 those for k = 2m + 1, so a strand's pairs stay in the wave that
+        // holds its vertices.  Both rest lengths a slot needs are adjacent.
+        constexpr int NPP = NPT / 2;
+        static_assert(NPT % 2 == 0, "pairs take two vertex slots");
+        int pairLv[NPP];
+        float2 restLen[NPP];
+#pragma unroll
+        for (int m = 0; m < NPP; ++m) {
+            pairLv[m] = (2 * m + ((t >> 5) & 1)) * kFusedBlock + (t & ~63) + 2 * (t & 31);
+            const int ls = pairLv[m] / V, v = pairLv[m] % V;
+            const size_t i = ((size_t)sIdx[ls < nb ? ls : 0] * (F + 1)) * V + v;
+            restLen[m] = ldg(reinterpret_cast<const float2*>(&h.restlen[i]));
         }
         const bool reset = P.reset != 0.0f;
         const float grav = P.gravity, dt = P.dt, decay = P.decay;
@@ -255,17 +270,20 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         // for the pass, and stores vertex i+1 as the step left it (step i+1
         // replaces it; after the last step it is vertex V-1's final value).
         // So up to kLscPipe passes run together, pass k on its own lane
-        // kLscLag = 2 steps behind pass k-1: pass k reads vertex i+1 at its
-        // step i, one step after pass k-1 stored it final at its step i+1,
-        // from when on pass k-1 stores only vertices above i+1; and pass k-1
-        // never reads a vertex that pass k has stored.  Every step reads
-        // exactly the values the sequential order gives it.  The lanes of one
-        // strand are adjacent in one wave; the wave runs the steps in lock
-        // step and orders its LDS accesses by a wave-scope fence per step.
+        // kLscLag = 3 steps behind pass k-1.  Step i's reads (vertex i+1 and
+        // the bind pose of i) are issued one step early, during step i-1:
+        // pass k-1 stored vertex i+1 final at its step i+1, one step before
+        // pass k's step i-1, and from then on stores only vertices above i+1;
+        // and pass k-1 never reads a vertex that pass k has stored.  Every
+        // step reads exactly the values the sequential order gives it, and
+        // its LDS reads are in flight while the step before it computes.  The
+        // lanes of one strand are adjacent in one wave; the wave runs the
+        // steps in lock step and orders its LDS accesses by a wave-scope fence
+        // per step.
         // The waves the sweep leaves idle compute P0 of the block's next work
         // item meanwhile (sMat and sSkinQ are dead from P4 on: P1 last read
         // them, and P2/P3 their VSP reuse).
-        constexpr int kLscPipe = 4, kLscLag = 2;
+        constexpr int kLscPipe = 4, kLscLag = 3;
         const int npass = P.lsc_passes > 0 ? P.lsc_passes : 0;
         // 1, 2 or 4 lanes per strand: a count that divides 64 keeps the
         // lanes of a strand in one wave (with 3 or more passes the extra
@@ -294,36 +312,50 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     const bool live = group + j < npass;
                     const int lag = kLscLag * j;
                     const int steps = (V - 2) + kLscLag * (min(lanes, npass - group) - 1);
-                    float4 pm = make_float4(0.f, 0.f, 0.f, 0.f), p = pm;
+                    // Every lane runs every iteration, so the loop body
+                    // branches only to start a pass: a lane outside its pass's
+                    // steps works on clamped indices and stores into the
+                    // strand's padding slot (index V), which no step uses.
+                    // Reads for step lv+1 are issued at step lv (for the first
+                    // step, before the loop); vertex 0 never moves.
+                    const float4 root = sp[0];
+                    int ln = min(max(1 - lag, 1), V - 2);
+                    float4 ppN = sp[ln + 1], vbwN = svb[ln];
+                    float2 nyzN = snb[ln];
+                    float4 pm = root, p = root;
                     for (int it = 1; it <= steps; ++it) {
                         const int lv = it - lag;
-                        if (live && lv >= 1 && lv <= V - 2) {
-                            if (lv == 1) { pm = sp[0]; p = sp[1]; }
-                            float4 pp = sp[lv + 1];
-                            const float4 vbw = svb[lv];
-                            const float2 nyz = snb[lv];
-                            const f3 vecBindPose = xyz(vbw);
-                            const f3 nLastBind{vbw.w, nyz.x, nyz.y};
-                            f3 lastVec = sub(xyz(p), xyz(pm));
-                            f3 org = add(rotate_from_to(nLastBind, normalize3(lastVec), vecBindPose), xyz(p));
+                        const bool act = live && lv >= 1 && lv <= V - 2;
+                        if (lv == 1) { pm = root; p = sp[1]; }
+                        float4 pp = ppN;
+                        const float4 vbw = vbwN;
+                        const float2 nyz = nyzN;
+                        ln = min(max(lv + 1, 1), V - 2);
+                        ppN = sp[ln + 1];
+                        vbwN = svb[ln];
+                        nyzN = snb[ln];
+                        const f3 vecBindPose = xyz(vbw);
+                        const f3 nLastBind{vbw.w, nyz.x, nyz.y};
 #ifndef TFX_FAST_BUILD
-                            f3 del = mul(stiff, sub(org, xyz(pp)));
-                            if (movable(p)) { p.x -= del.x; p.y -= del.y; p.z -= del.z; }
-                            if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
+                        f3 lastVec = sub(xyz(p), xyz(pm));
+                        f3 org = add(rotate_from_to(nLastBind, normalize3(lastVec), vecBindPose), xyz(p));
+                        f3 del = mul(stiff, sub(org, xyz(pp)));
+                        if (movable(p)) { p.x -= del.x; p.y -= del.y; p.z -= del.z; }
+                        if (movable(pp)) { pp.x += del.x; pp.y += del.y; pp.z += del.z; }
 #else
-                            // the stiffness of an immovable vertex is zero
-                            const f3 dif = sub(org, xyz(pp));
-                            const float sp0 = movable(p) ? stiff : 0.0f, sp1 = movable(pp) ? stiff : 0.0f;
-                            p.x -= sp0 * dif.x; p.y -= sp0 * dif.y; p.z -= sp0 * dif.z;
-                            pp.x += sp1 * dif.x; pp.y += sp1 * dif.y; pp.z += sp1 * dif.z;
+                        const f3 org = lsc_target(nLastBind, vecBindPose, xyz(pm), xyz(p));
+                        // the stiffness of an immovable vertex is zero
+                        const f3 dif = sub(org, xyz(pp));
+                        const float sp0 = movable(p) ? stiff : 0.0f, sp1 = movable(pp) ? stiff : 0.0f;
+                        p.x -= sp0 * dif.x; p.y -= sp0 * dif.y; p.z -= sp0 * dif.z;
+                        pp.x += sp1 * dif.x; pp.y += sp1 * dif.y; pp.z += sp1 * dif.z;
 #endif
-                            // vertex lv is final for this pass; lv+1 is final
-                            // once the next step has moved it (or, after the
-                            // last step, now: vertex V-1)
-                            sp[lv] = p;
-                            sp[lv + 1] = pp;
-                            pm = p; p = pp;
-                        }
+                        // vertex lv is final for this pass; lv+1 is final
+                        // once the next step has moved it (or, after the
+                        // last step, now: vertex V-1)
+                        sp[act ? lv : V] = p;
+                        sp[act ? lv + 1 : V] = pp;
+                        pm = p; p = pp;
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -370,25 +402,25 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             }
             wave_sync();
         }
+        // Red-black length constraints: the even pairs (v, v+1), then the
+        // odd ones, each pair on its own lane.
         for (int it = 0; it < P.length_iters; ++it) {
 #pragma unroll
-            for (int k = 0; k < NPT; ++k) {
-                const int lv = k * kFusedBlock + t;
-                const int ls = lv / V, v = lv % V;
-                if (ls < nb && (v & 1) == 0 && (v >> 1) < V / 2) {
+            for (int m = 0; m < NPP; ++m) {
+                const int ls = pairLv[m] / V, v = pairLv[m] % V;
+                if (ls < nb) {
                     float4 a = sPos[ls * STRIDE + v], b = sPos[ls * STRIDE + v + 1];
-                    distance_constraint(a, b, restLen[k]);
+                    distance_constraint(a, b, restLen[m].x);
                     sPos[ls * STRIDE + v] = a; sPos[ls * STRIDE + v + 1] = b;
                 }
             }
             wave_sync();
 #pragma unroll
-            for (int k = 0; k < NPT; ++k) {
-                const int lv = k * kFusedBlock + t;
-                const int ls = lv / V, v = lv % V;
-                if (ls < nb && (v & 1) == 1 && (v >> 1) < (V - 1) / 2) {
+            for (int m = 0; m < NPP; ++m) {
+                const int ls = pairLv[m] / V, v = pairLv[m] % V + 1;
+                if (ls < nb && (v >> 1) < (V - 1) / 2) {
                     float4 a = sPos[ls * STRIDE + v], b = sPos[ls * STRIDE + v + 1];
-                    distance_constraint(a, b, restLen[k]);
+                    distance_constraint(a, b, restLen[m].y);
                     sPos[ls * STRIDE + v] = a; sPos[ls * STRIDE + v + 1] = b;
                 }
             }
