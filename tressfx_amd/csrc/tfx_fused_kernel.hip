@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     float4 ppN = sp[ln + 1], vbwN = svb[ln];
                     float2 nyzN = snb[ln];
                     float4 pm = root, p = root;
-                    for (int it = 1; it <= steps; ++it) {
+                    auto step = [&](const int it) {
                         const int lv = it - lag;
                         const bool act = live && lv >= 1 && lv <= V - 2;
                         if (lv == 1) { pm = root; p = sp[1]; }
@@ -359,7 +359,17 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    };
+                    // four steps per trip: the compiler renames the values a
+                    // step hands to the next instead of copying them
+                    int it = 1;
+                    for (; it + 3 <= steps; it += 4) {
+                        step(it);
+                        step(it + 1);
+                        step(it + 2);
+                        step(it + 3);
                     }
+                    for (; it <= steps; ++it) step(it);
                 }
                 __builtin_amdgcn_s_setprio(0);
             }
