@@ -7,10 +7,11 @@
 // rotated-sign bit pattern orders by |d| with ties going to +d, so the grid is
 // bit-identical for any triangle order or lane mapping.
 //
-// Construct maps a team of lanes to one triangle and spreads the triangle's
-// cell box over the team (instead of one thread looping over the whole box), so
-// large and small triangles share a wave without one lane serialising it; a
-// node's atomic is skipped when its current value is already no larger.
+// Construct maps a team of lanes to one triangle and spreads the z-columns of
+// the triangle's cell box over the team (instead of one thread looping over
+// the whole box), so large and small triangles share a wave without one lane
+// serialising it; a node's atomic is skipped when its current value is
+// already no larger.
 #include "tfx_device.h"
 
 namespace tfx {
@@ -67,37 +68,60 @@ __device__ __forceinline__ TriPrep tri_prep(f3 x0, f3 x1, f3 x2) {
     return t;
 }
 
-// The squared length DistancePointToEdge takes the square root of, with the
-// edge x10 = x1 - x0, its squared length q and 1/q precomputed.
-__device__ __forceinline__ float sq_distance_point_to_edge(f3 p, f3 x0, f3 x1, f3 x10, float q, float rq) {
-    float t = div_fixed(dot3(sub(x1, p), x10), q, rq);
+// The six dot products SignedDistancePointToTriangle (:163-215) takes of the
+// query point p: with px2 = p - x2, x12n.px2 and x02n.px2 (the barycentric
+// frame), (x1-p).e10, (x2-p).e21 and (x2-p).e20 (DistancePointToEdge's
+// parameter along each edge, before the division), and (p-x0).nTri (the
+// side).  Each is (X + Y) + Z of one per-axis term per coordinate, a term
+// depending on that coordinate of p alone.
+struct TriDots {
+    float b, a, e10, e21, e20, side;
+};
+__device__ __forceinline__ void tri_dot_terms(const TriPrep& t, float pv, int k, float (&o)[6]) {
+    const float c0 = k == 0 ? t.x0.x : k == 1 ? t.x0.y : t.x0.z;
+    const float c1 = k == 0 ? t.x1.x : k == 1 ? t.x1.y : t.x1.z;
+    const float c2 = k == 0 ? t.x2.x : k == 1 ? t.x2.y : t.x2.z;
+    const float px2 = pv - c2;
+    o[0] = (k == 0 ? t.x12n.x : k == 1 ? t.x12n.y : t.x12n.z) * px2;
+    o[1] = (k == 0 ? t.x02n.x : k == 1 ? t.x02n.y : t.x02n.z) * px2;
+    o[2] = (c1 - pv) * (k == 0 ? t.e10.x : k == 1 ? t.e10.y : t.e10.z);
+    o[3] = (c2 - pv) * (k == 0 ? t.e21.x : k == 1 ? t.e21.y : t.e21.z);
+    o[4] = (c2 - pv) * (k == 0 ? t.e20.x : k == 1 ? t.e20.y : t.e20.z);
+    o[5] = (pv - c0) * (k == 0 ? t.nTri.x : k == 1 ? t.nTri.y : t.nTri.z);
+}
+
+// The squared length DistancePointToEdge takes the square root of, given
+// the edge parameter's numerator u = (x1 - p).x10.
+__device__ __forceinline__ float sq_distance_point_to_edge_u(f3 p, f3 x0, f3 x1, float u, float q, float rq) {
+    float t = div_fixed(u, q, rq);
     t = fmx(0.0f, fmn(t, 1.0f));
     f3 a = sub(p, add(mul(t, x0), mul(1.0f - t, x1)));
     return dot3(a, a);
 }
 
-// SignedDistancePointToTriangle :163-215.  Outside the face region the
-// reference takes min(sqrt(e01), sqrt(e12), sqrt(e02)) of the three squared
-// edge distances; the correctly rounded square root is monotone, so
-// sqrt(min(e01, e12, e02)) is the same value (also for infinities and NaNs,
-// which fmin drops) with one square root instead of three.
-__device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p) {
-    f3 px2 = sub(p, t.x2);
-    float b = div_fixed(dot3(t.x12n, px2), t.l2, t.rl2);
-    float a = div_fixed(dot3(t.x02n, px2) - t.l1 * b, t.l0, t.rl0);
+// SignedDistancePointToTriangle :163-215 from the point and its six dot
+// products.  Outside the face region the reference takes min(sqrt(e01),
+// sqrt(e12), sqrt(e02)) of the three squared edge distances; the correctly
+// rounded square root is monotone, so sqrt(min(e01, e12, e02)) is the same
+// value (also for infinities and NaNs, which fmin drops) with one square root
+// instead of three; the face region's length is the square root of its
+// squared length too, so one square root serves both.
+__device__ __forceinline__ float signed_distance(const TriPrep& t, f3 p, const TriDots& dt) {
+    float b = div_fixed(dt.b, t.l2, t.rl2);
+    float a = div_fixed(dt.a - t.l1 * b, t.l0, t.rl0);
     float c = (1.0f - a) - b;
     const float tol = 1e-8f;
-    float d;
+    float e;
     if (a >= -tol && b >= -tol && c >= -tol) {
         f3 n = sub(p, add(add(mul(a, t.x0), mul(b, t.x1)), mul(c, t.x2)));
-        d = len3(n);
+        e = dot3(n, n);
     } else {
-        float e = sq_distance_point_to_edge(p, t.x0, t.x1, t.e10, t.q10, t.rq10);
-        e = fmn(e, sq_distance_point_to_edge(p, t.x1, t.x2, t.e21, t.q21, t.rq21));
-        e = fmn(e, sq_distance_point_to_edge(p, t.x0, t.x2, t.e20, t.q20, t.rq20));
-        d = sqrtf(e);
+        e = sq_distance_point_to_edge_u(p, t.x0, t.x1, dt.e10, t.q10, t.rq10);
+        e = fmn(e, sq_distance_point_to_edge_u(p, t.x1, t.x2, dt.e21, t.q21, t.rq21));
+        e = fmn(e, sq_distance_point_to_edge_u(p, t.x0, t.x2, dt.e20, t.q20, t.rq20));
     }
-    return (dot3(sub(p, t.x0), t.nTri) < 0.0f) ? -d : d;
+    const float d = sqrtf(e);
+    return (dt.side < 0.0f) ? -d : d;
 }
 
 // i / n and i % n for 0 <= i < 2^24, n >= 1: the float estimate is within one
@@ -265,59 +289,156 @@ __device__ __forceinline__ void triangle_node_box(const tfx_standard_vertex* ver
     }
 }
 
-// ConstructSignedDistanceField :329-397 for one triangle, spread over the
-// kTeam lanes of a team: the triangle's padded node box walked in x-fastest
-// order.  A node's current value is read first (agent-coherent load) and the
-// atomicMin issued only if the new value is smaller: the field only
-// decreases, so a value read at any time bounds the final one from above and
-// skipping is exact.
-template <int kTeam>
-__device__ __forceinline__ void construct_triangle(uint32_t* __restrict__ cells, const tfx_standard_vertex* verts,
-                                                   const uint32_t* idx, int tri, const SdfGrid& g, int lane) {
-    f3 x0, x1, x2;
-    int lo[3], hi[3];
-    triangle_node_box(verts, idx, tri, g, x0, x1, x2, lo, hi);
-    const TriPrep tp = tri_prep(x0, x1, x2);
+// The nodes of one triangle's padded box (ConstructSignedDistanceField
+// :329-397), spread over the kTeam lanes of a team: each lane walks whole
+// z-columns of the box, with the x and y terms of the six dot products (their
+// partial sums come first in the reference's order) formed once per column
+// and the z terms once per node.  `sink(ci, local, f)` takes each node's
+// FloatFlip3(signed distance): ci its grid index (GetSdfCellIndex :123-129,
+// wrapping int arithmetic), local its index in the box `ub` (x fastest, extents
+// uex, uey), which must contain this box.
+template <int kTeam, typename Sink>
+__device__ __forceinline__ void construct_nodes(const TriPrep& tp, const int (&lo)[3], const int (&hi)[3],
+                                                const SdfGrid& g, int lane, const int (&ub)[3], int uex, int uey,
+                                                Sink&& sink) {
     // An axis whose wrapped lower bound passes the upper one (a vertex more
     // than 2^31 cells below the origin) has no nodes, as in the reference's
     // nested loops.
-    const int ex = max(hi[0] - lo[0] + 1, 0), ey = max(hi[1] - lo[1] + 1, 0), ez = max(hi[2] - lo[2] + 1, 0);
-    const int total = ex * ey * ez;
+    const int ex = max(hi[0] - lo[0] + 1, 0), ey = max(hi[1] - lo[1] + 1, 0);
+    const int ncol = ex * ey;
     // Exact small-integer division through the float reciprocal: for
     // 0 <= c < 2^16, (c + 0.5) / e stays more than 0.5/e - 2^-8/e from any
     // integer, beyond the rounding of the two float operations.
-    const bool fast = total < 65536;
-    const float inv_x = 1.0f / (float)ex, inv_y = 1.0f / (float)ey;
-    for (int c = lane; c < total; c += kTeam) {
-        int rem, xx, yy, zz;
+    const bool fast = ncol < 65536;
+    const float inv_x = 1.0f / (float)ex;
+    const uint32_t plane = (uint32_t)g.nx * (uint32_t)g.ny;
+    const int uplane = uex * uey;
+    for (int c = lane; c < ncol; c += kTeam) {
+        int xx, yy;
         if (fast) {
-            rem = (int)(((float)c + 0.5f) * inv_x);
-            xx = c - rem * ex;
-            zz = (int)(((float)rem + 0.5f) * inv_y);
-            yy = rem - zz * ey;
+            yy = (int)(((float)c + 0.5f) * inv_x);
+            xx = c - yy * ex;
         } else {
             xx = c % ex;
-            rem = c / ex;
-            yy = rem % ey;
-            zz = rem / ey;
+            yy = c / ex;
         }
-        const int x = lo[0] + xx, y = lo[1] + yy, z = lo[2] + zz;
-        const float d = signed_distance(tp, sdf_cell_pos(g, x, y, z));
-        const uint32_t f = float_flip3(d);
-        TFX_ADDRSPACE(1) uint32_t* cell = (TFX_ADDRSPACE(1) uint32_t*)&cells[sdf_index(g, x, y, z)];
-        if (f < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            __hip_atomic_fetch_min(cell, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int x = lo[0] + xx, y = lo[1] + yy;
+        const float px = (float)x * g.cell + g.ox, py = (float)y * g.cell + g.oy;
+        float tx[6], ty[6], xy[6];
+        tri_dot_terms(tp, px, 0, tx);
+        tri_dot_terms(tp, py, 1, ty);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) xy[k] = tx[k] + ty[k];
+        uint32_t ci = plane * (uint32_t)lo[2] + (uint32_t)g.nx * (uint32_t)y + (uint32_t)x;
+        int local = ((lo[2] - ub[2]) * uey + (y - ub[1])) * uex + (x - ub[0]);
+        for (int z = lo[2]; z <= hi[2]; ++z, ci += plane, local += uplane) {
+            const float pz = (float)z * g.cell + g.oz;
+            float tz[6];
+            tri_dot_terms(tp, pz, 2, tz);
+            const TriDots dt{xy[0] + tz[0], xy[1] + tz[1], xy[2] + tz[2], xy[3] + tz[3], xy[4] + tz[4], xy[5] + tz[5]};
+            sink(ci, local, float_flip3(signed_distance(tp, f3{px, py, pz}, dt)));
+        }
     }
 }
 
-// ConstructSignedDistanceField :329-397: kTeam lanes per triangle.
+// atomicMin of FloatFlip3 bits into a grid node, issued only if the node's
+// current value (agent-coherent load) is larger: the field only decreases, so
+// a value read at any time bounds the final one from above and skipping is
+// exact.
+__device__ __forceinline__ void grid_min(uint32_t* __restrict__ cells, uint32_t ci, uint32_t f) {
+    TFX_ADDRSPACE(1) uint32_t* cell = (TFX_ADDRSPACE(1) uint32_t*)&cells[(int)ci];
+    if (f < __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        __hip_atomic_fetch_min(cell, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Nodes (4 bytes each) of the block's LDS staging box.
+constexpr int kStageNodes = 6144;
+
+// One round of ConstructSignedDistanceField :329-397 for the block's
+// kSdfBlock / kTeam consecutive triangles tri0, tri0 + 1, ... (a team each).
+// Neighbouring triangles of a mesh usually follow each other in its index
+// buffer, so their padded boxes overlap: when the union of the round's boxes
+// holds at most kStageNodes nodes, every node's minimum over the round is
+// first taken in LDS (atomicMin on the same FloatFlip3 bits) and each touched
+// node then goes to the grid once; otherwise every node goes to the grid
+// directly.  Either way each grid node ends as the minimum over all
+// triangles, so the grid is the same.  Called by every thread of the block.
+template <int kTeam>
+__device__ __forceinline__ void construct_round(uint32_t* __restrict__ cells, const tfx_standard_vertex* verts,
+                                                const uint32_t* idx, int tri0, int ntri, const SdfGrid& g,
+                                                uint32_t* sStage, int* sBox) {
+    const int t = threadIdx.x, lane = t % kTeam, tri = tri0 + t / kTeam;
+    if (t < 3) {
+        sBox[t] = 0x7fffffff;
+        sBox[3 + t] = -0x7fffffff - 1;
+    }
+    __syncthreads();
+    f3 x0, x1, x2;
+    int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+    const bool valid = tri < ntri;
+    if (valid) triangle_node_box(verts, idx, tri, g, x0, x1, x2, lo, hi);
+    const bool nodes = valid && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+    if (nodes && lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&sBox[k], lo[k]);
+            atomicMax(&sBox[3 + k], hi[k]);
+        }
+    }
+    __syncthreads();
+    // the union box (extents clamped, so that the volume cannot overflow);
+    // staged only if no team is left out of it
+    int ub[3], ue[3];
+    long long vol = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ub[k] = sBox[k];
+        const long long e = (long long)sBox[3 + k] - (long long)ub[k] + 1;
+        ue[k] = (int)min(max(e, 0LL), (long long)kStageNodes + 1);
+        vol *= ue[k];
+    }
+    const bool staged = vol > 0 && vol <= kStageNodes;
+    if (staged) {
+        for (int n = t; n < (int)vol; n += kSdfBlock) sStage[n] = 0xffffffffu;
+        __syncthreads();
+    }
+    if (nodes) {
+        const TriPrep tp = tri_prep(x0, x1, x2);
+        if (staged)
+            construct_nodes<kTeam>(tp, lo, hi, g, lane, ub, ue[0], ue[1],
+                                   [&](uint32_t, int local, uint32_t f) { atomicMin(&sStage[local], f); });
+        else
+            construct_nodes<kTeam>(tp, lo, hi, g, lane, lo, 0, 0,
+                                   [&](uint32_t ci, int, uint32_t f) { grid_min(cells, ci, f); });
+    }
+    if (staged) {
+        __syncthreads();
+        const uint32_t plane = (uint32_t)g.nx * (uint32_t)g.ny;
+        const int uplane = ue[0] * ue[1];
+        const float inv_x = 1.0f / (float)ue[0], inv_p = 1.0f / (float)uplane;
+        for (int n = t; n < (int)vol; n += kSdfBlock) {
+            const uint32_t f = sStage[n];
+            if (f == 0xffffffffu) continue;
+            // n < kStageNodes < 2^16: exact float-reciprocal division
+            const int zz = (int)(((float)n + 0.5f) * inv_p), r = n - zz * uplane;
+            const int yy = (int)(((float)r + 0.5f) * inv_x), xx = r - yy * ue[0];
+            const uint32_t ci = plane * (uint32_t)(ub[2] + zz) + (uint32_t)g.nx * (uint32_t)(ub[1] + yy) +
+                                (uint32_t)(ub[0] + xx);
+            grid_min(cells, ci, f);
+        }
+    }
+    __syncthreads();  // the next round resets sBox and reuses sStage
+}
+
+// ConstructSignedDistanceField :329-397: kTeam lanes per triangle, one round
+// of consecutive triangles per block.
 template <int kTeam>
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct(uint32_t* __restrict__ sdf,
                                                              const tfx_standard_vertex* __restrict__ verts,
                                                              const uint32_t* __restrict__ idx, int ntri, SdfGrid g) {
-    const int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam;
-    if (tri >= ntri) return;
-    construct_triangle<kTeam>(sdf, verts, idx, tri, g, threadIdx.x % kTeam);
+    __shared__ uint32_t sStage[kStageNodes];
+    __shared__ int sBox[6];
+    construct_round<kTeam>(sdf, verts, idx, blockIdx.x * (kSdfBlock / kTeam), ntri, g, sStage, sBox);
 }
 
 // CollideHairVerticesWithSdf_forward :531-595 — one lane per vertex, guide
@@ -508,11 +629,13 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_init_multi(const SdfJob* __re
 
 template <int kTeam>
 __global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob* __restrict__ jobs) {
+    __shared__ uint32_t sStage[kStageNodes];
+    __shared__ int sBox[6];
     const SdfJob& j = jobs[blockIdx.y];
     const SdfGrid g = j.g;
-    const int lane = threadIdx.x % kTeam;
-    for (int tri = (blockIdx.x * kSdfBlock + threadIdx.x) / kTeam; tri < j.ntri; tri += gridDim.x * (kSdfBlock / kTeam))
-        construct_triangle<kTeam>(j.cells, j.verts, j.idx, tri, g, lane);
+    constexpr int kPerRound = kSdfBlock / kTeam;
+    for (int tri0 = blockIdx.x * kPerRound; tri0 < j.ntri; tri0 += gridDim.x * kPerRound)
+        construct_round<kTeam>(j.cells, j.verts, j.idx, tri0, j.ntri, g, sStage, sBox);
 }
 
 // Finalize of quads part, part + kL, ... of brick (bx, by, bz) (the brick's
