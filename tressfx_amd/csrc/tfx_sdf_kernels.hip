@@ -488,7 +488,7 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_phase)(int phase, uint32_t* sdf, i
             hipLaunchKernelGGL(k_sdf_init, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell, bricks, nbrick);
     } else if (phase == 1) {
         if (ntri > 0) {
-            constexpr int kTeam = 16;
+            constexpr int kTeam = 8;
             const long long lanes = (long long)ntri * kTeam;
             const int tblocks = (int)((lanes + kSdfBlock - 1) / kSdfBlock);
             hipLaunchKernelGGL(k_sdf_construct<kTeam>, dim3(tblocks), dim3(kSdfBlock), 0, stream, sdf, verts, idx, ntri,
@@ -771,7 +771,7 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_multi)(int phase, const tfx::SdfJo
             }
             break;
         case 1: {
-            constexpr int kTeam = 16;
+            constexpr int kTeam = 8;
             const int tx = std::max(1, std::min((max_tris * kTeam + kSdfBlock - 1) / kSdfBlock, 4096));
             hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kSdfBlock), 0, stream, jobs);
             break;
