@@ -40,6 +40,58 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// sdf_collide_batch for the U vertices of each lane of a wave, with the
+// projections of the vertices inside the margin packed onto consecutive
+// lanes: the distance lookups of all U slots go out together, then the lanes
+// of the wave take the wave's hits one each (64 at a time), so every hit's
+// gradient lookup is in flight at once and each lane projects at most one
+// vertex per round, instead of one gradient round trip per slot.  `wl`: 64
+// float4 of LDS private to the calling wave.  Every lane of the wave calls
+// it.  Per vertex the same lookups in the same arithmetic as
+// sdf_collide_batch.
+template <int U>
+__device__ __forceinline__ void sdf_collide_wave(const SdfGrid& g, const float* __restrict__ sdf, float margin,
+                                                 const f3 (&p)[U], const bool (&act)[U], f3 (&out)[U],
+                                                 bool (&hit)[U], float4* wl) {
+    float d[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) d[u] = (act[u] && sdf_maybe_near(g, p[u])) ? sdf_distance_t<true>(g, sdf, p[u]) : 1e10f;
+    bool want[U];
+    int pos[U];
+    int total = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        hit[u] = false;
+        out[u] = p[u];
+        want[u] = act[u] && !(d[u] > margin);
+        const unsigned long long m = __ballot(want[u]);
+        pos[u] = total + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        total += __popcll(m);
+    }
+    const int lane = __lane_id();
+    for (int base = 0; base < total; base += 64) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (want[u] && pos[u] >= base && pos[u] < base + 64) wl[pos[u] - base] = make_float4(p[u].x, p[u].y, p[u].z, d[u]);
+        wave_sync();
+        if (base + lane < total) {
+            const float4 e = wl[lane];
+            f3 o;
+            const bool h = sdf_project<true>(g, sdf, margin, xyz(e), e.w, o);
+            wl[lane] = make_float4(o.x, o.y, o.z, h ? 1.0f : 0.0f);
+        }
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (want[u] && pos[u] >= base && pos[u] < base + 64) {
+                const float4 r = wl[pos[u] - base];
+                hit[u] = r.w != 0.0f;
+                if (hit[u]) out[u] = xyz(r);
+            }
+        wave_sync();
+    }
+}
+
 template <int V, int VB>
 struct FusedShape {
     static constexpr int B = VB / V;                 // guide strands per block
@@ -63,7 +115,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     //   in (sVB[i].w, sNB[i].x, sNB[i].y), bind = MultQuaternionAndVector(skinningQuat, rest).
     //   P6: sVB carries the guide tangents.
     __shared__ float4 sVB[B * STRIDE];
-    __shared__ float2 sNB[B * STRIDE];
+    __shared__ __align__(16) float2 sNB[B * STRIDE];
     __shared__ float sMat[B][12];      // P0->P1 blended bone; P2->P3 vsp quat (0..3), translation+coeff (4..7)
     __shared__ float4 sSkinQ[B];       // P0->P1 skinning quaternion
     __shared__ float4 sV01[B][3];      // P1->P2 prev(v0), prev(v1), prevprev(v1)
@@ -477,12 +529,11 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         const float C = P.clamp;
         const bool tw_uniform = P.tangent_w_uniform != 0;
         const float* __restrict__ sdfp = h.sdf;
-        constexpr int U = NPT < 4 ? NPT : 4;
+        constexpr int U = 2;
 #pragma unroll
         for (int k0 = 0; k0 < NPT; k0 += U) {
             f3 cp[U], outp[U];
             bool act[U], hit[U];
-            float4 pvs[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = k0 + u;
@@ -490,7 +541,6 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 const int ls = lv / V, v = lv % V;
                 act[u] = false;
                 cp[u] = f3{0.f, 0.f, 0.f};
-                pvs[u] = prevReg[k];
                 if (ls < nb) {
                     const float4 sp = sPos[ls * STRIDE + v];
                     float4 pv = prevReg[k];
@@ -503,12 +553,14 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                         pv.x = np.x; pv.y = np.y; pv.z = np.z;
                     }
                     if (hitMask & (1u << k)) pv = sp;
-                    pvs[u] = pv;
+                    prevReg[k] = pv;  // the previous position the vertex keeps
                     cp[u] = xyz(sp);
                     act[u] = has_sdf && v >= 2 && maybe_near(cp[u]);
                 }
             }
-            sdf_collide_batch<U>(grid, sdfp, margin, cp, act, outp, hit);
+            // sNB is dead from P4 on: each wave packs its projections in a
+            // 64-float4 part of it (B * STRIDE >= VB + B >= 512 slots)
+            sdf_collide_wave<U>(grid, sdfp, margin, cp, act, outp, hit, reinterpret_cast<float4*>(sNB) + (t >> 6) * 64);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = k0 + u;
@@ -520,7 +572,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                     const int ti = (v == V - 1) ? v - 1 : v;
                     const f3 tan = normalize3(sub(xyz(sPos[base + ti + 1]), xyz(sPos[base + ti])));
                     float4 pout = sPos[base + v];
-                    float4 pv = pvs[u];
+                    float4 pv = prevReg[k];
                     if (hit[u]) {
                         pout.x = outp[u].x; pout.y = outp[u].y; pout.z = outp[u].z;
                         pv.x = outp[u].x; pv.y = outp[u].y; pv.z = outp[u].z;
