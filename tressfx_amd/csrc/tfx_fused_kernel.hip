@@ -92,6 +92,15 @@ __device__ __forceinline__ void sdf_collide_wave(const SdfGrid& g, const float* 
     }
 }
 
+// x of lane + 1 and of lane - 1 of the wave (DPP wave_shl:1 / wave_shr:1;
+// the last and the first lane get 0).
+__device__ __forceinline__ float from_next_lane(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_prev_lane(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
+}
+
 template <int V, int VB>
 struct FusedShape {
     static constexpr int B = VB / V;                 // guide strands per block
@@ -154,6 +163,8 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     // Whether the current item's P0 was done ahead, during the previous
     // item's LocalShapeConstraints phase.
     bool blended = false;
+    // The occupancy sBits holds (null: none).
+    const uint32_t* bits_in_lds = nullptr;
 
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const WorkItem item = items[w];
@@ -172,8 +183,12 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         const bool has_sdf = P.has_sdf != 0 && !defer_sdf;
         const int brick_words = (P.sdf_bx * P.sdf_by * P.sdf_bz + 31) >> 5;
         const bool bits_lds = has_sdf && h.sdf_brick_bits != nullptr && brick_words <= kMaxBrickWords;
-        if (bits_lds)
+        // (kept from the block's previous item when that item read the same
+        // occupancy: a block's successive items mostly belong to one object,
+        // and no launch writes the occupancy it reads)
+        if (bits_lds && h.sdf_brick_bits != bits_in_lds)
             for (int k = t; k < brick_words; k += kFusedBlock) sBits[k] = ldg(&h.sdf_brick_bits[k]);
+        bits_in_lds = bits_lds ? h.sdf_brick_bits : nullptr;
 
         // ---- P0: blended bone per strand ---------------------------------
         const int* ord = order_of(h);
@@ -232,10 +247,12 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             float4 initialPos = valid ? restIn[k] : make_float4(0.f, 0.f, 0.f, 0.f);
             // Bind pose of LocalShapeConstraints, MultQuaternionAndVector(boneQuat,
             // rest) (:799-801); a strand's vertices sit in consecutive lanes of one
-            // wave (64 % V == 0), so the neighbours come from lane +-1.
+            // wave (64 % V == 0), so the neighbours come from lane +-1 (a
+            // wave-wide DPP shift; the values the first and last lanes of the
+            // wave get are not used).
             const f3 bind = qrot(sSkinQ[valid ? ls : 0], xyz(initialPos));
-            const f3 bindP{__shfl_down(bind.x, 1), __shfl_down(bind.y, 1), __shfl_down(bind.z, 1)};
-            const f3 bindM{__shfl_up(bind.x, 1), __shfl_up(bind.y, 1), __shfl_up(bind.z, 1)};
+            const f3 bindP{from_next_lane(bind.x), from_next_lane(bind.y), from_next_lane(bind.z)};
+            const f3 bindM{from_prev_lane(bind.x), from_prev_lane(bind.y), from_prev_lane(bind.z)};
             if (valid) {
                 const f3 vb = sub(bindP, bind);
                 f3 nbp{0.f, 0.f, 0.f};
