@@ -18,6 +18,7 @@ namespace tfx {
 inline namespace TFX_KERNEL_NS {
 
 constexpr int kSdfBlock = 256;
+constexpr int kConstructBlock = 256;  // threads of a construct block (one team per triangle)
 
 // FloatFlip3 / IFloatFlip3 :92-100
 __device__ __forceinline__ uint32_t float_flip3(float f) {
@@ -355,7 +356,7 @@ __device__ __forceinline__ void grid_min(uint32_t* __restrict__ cells, uint32_t 
 constexpr int kStageNodes = 6144;
 
 // One round of ConstructSignedDistanceField :329-397 for the block's
-// kSdfBlock / kTeam consecutive triangles tri0, tri0 + 1, ... (a team each).
+// kConstructBlock / kTeam consecutive triangles tri0, tri0 + 1, ... (a team each).
 // Neighbouring triangles of a mesh usually follow each other in its index
 // buffer, so their padded boxes overlap: when the union of the round's boxes
 // holds at most kStageNodes nodes, every node's minimum over the round is
@@ -399,7 +400,7 @@ __device__ __forceinline__ void construct_round(uint32_t* __restrict__ cells, co
     }
     const bool staged = vol > 0 && vol <= kStageNodes;
     if (staged) {
-        for (int n = t; n < (int)vol; n += kSdfBlock) sStage[n] = 0xffffffffu;
+        for (int n = t; n < (int)vol; n += kConstructBlock) sStage[n] = 0xffffffffu;
         __syncthreads();
     }
     if (nodes) {
@@ -416,7 +417,7 @@ __device__ __forceinline__ void construct_round(uint32_t* __restrict__ cells, co
         const uint32_t plane = (uint32_t)g.nx * (uint32_t)g.ny;
         const int uplane = ue[0] * ue[1];
         const float inv_x = 1.0f / (float)ue[0], inv_p = 1.0f / (float)uplane;
-        for (int n = t; n < (int)vol; n += kSdfBlock) {
+        for (int n = t; n < (int)vol; n += kConstructBlock) {
             const uint32_t f = sStage[n];
             if (f == 0xffffffffu) continue;
             // n < kStageNodes < 2^16: exact float-reciprocal division
@@ -433,12 +434,12 @@ __device__ __forceinline__ void construct_round(uint32_t* __restrict__ cells, co
 // ConstructSignedDistanceField :329-397: kTeam lanes per triangle, one round
 // of consecutive triangles per block.
 template <int kTeam>
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_construct(uint32_t* __restrict__ sdf,
+__global__ void __launch_bounds__(kConstructBlock) k_sdf_construct(uint32_t* __restrict__ sdf,
                                                              const tfx_standard_vertex* __restrict__ verts,
                                                              const uint32_t* __restrict__ idx, int ntri, SdfGrid g) {
     __shared__ uint32_t sStage[kStageNodes];
     __shared__ int sBox[6];
-    construct_round<kTeam>(sdf, verts, idx, blockIdx.x * (kSdfBlock / kTeam), ntri, g, sStage, sBox);
+    construct_round<kTeam>(sdf, verts, idx, blockIdx.x * (kConstructBlock / kTeam), ntri, g, sStage, sBox);
 }
 
 // CollideHairVerticesWithSdf_forward :531-595 — one lane per vertex, guide
@@ -490,8 +491,8 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_phase)(int phase, uint32_t* sdf, i
         if (ntri > 0) {
             constexpr int kTeam = 8;
             const long long lanes = (long long)ntri * kTeam;
-            const int tblocks = (int)((lanes + kSdfBlock - 1) / kSdfBlock);
-            hipLaunchKernelGGL(k_sdf_construct<kTeam>, dim3(tblocks), dim3(kSdfBlock), 0, stream, sdf, verts, idx, ntri,
+            const int tblocks = (int)((lanes + kConstructBlock - 1) / kConstructBlock);
+            hipLaunchKernelGGL(k_sdf_construct<kTeam>, dim3(tblocks), dim3(kConstructBlock), 0, stream, sdf, verts, idx, ntri,
                                g);
         }
     } else if (phase == 2) {
@@ -628,12 +629,12 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf_init_multi(const SdfJob* __re
 }
 
 template <int kTeam>
-__global__ void __launch_bounds__(kSdfBlock) k_sdf_construct_multi(const SdfJob* __restrict__ jobs) {
+__global__ void __launch_bounds__(kConstructBlock) k_sdf_construct_multi(const SdfJob* __restrict__ jobs) {
     __shared__ uint32_t sStage[kStageNodes];
     __shared__ int sBox[6];
     const SdfJob& j = jobs[blockIdx.y];
     const SdfGrid g = j.g;
-    constexpr int kPerRound = kSdfBlock / kTeam;
+    constexpr int kPerRound = kConstructBlock / kTeam;
     for (int tri0 = blockIdx.x * kPerRound; tri0 < j.ntri; tri0 += gridDim.x * kPerRound)
         construct_round<kTeam>(j.cells, j.verts, j.idx, tri0, j.ntri, g, sStage, sBox);
 }
@@ -772,8 +773,8 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_multi)(int phase, const tfx::SdfJo
             break;
         case 1: {
             constexpr int kTeam = 8;
-            const int tx = std::max(1, std::min((max_tris * kTeam + kSdfBlock - 1) / kSdfBlock, 4096));
-            hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kSdfBlock), 0, stream, jobs);
+            const int tx = std::max(1, std::min((max_tris * kTeam + kConstructBlock - 1) / kConstructBlock, 4096));
+            hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kConstructBlock), 0, stream, jobs);
             break;
         }
         case 2:
