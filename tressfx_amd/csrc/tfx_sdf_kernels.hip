@@ -489,7 +489,9 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_phase)(int phase, uint32_t* sdf, i
             hipLaunchKernelGGL(k_sdf_init, dim3(cblocks), dim3(kSdfBlock), 0, stream, sdf, ncell, bricks, nbrick);
     } else if (phase == 1) {
         if (ntri > 0) {
-            constexpr int kTeam = 8;
+            // one grid per launch: 16 lanes per triangle keep enough blocks
+            // in flight for a single mesh (the batched build takes 8)
+            constexpr int kTeam = 16;
             const long long lanes = (long long)ntri * kTeam;
             const int tblocks = (int)((lanes + kConstructBlock - 1) / kConstructBlock);
             hipLaunchKernelGGL(k_sdf_construct<kTeam>, dim3(tblocks), dim3(kConstructBlock), 0, stream, sdf, verts, idx, ntri,
