@@ -547,10 +547,12 @@ def main():
     from tressfx_amd import synth
 
     host_workers = max(1, min(16, cpu_threads()[0]))
-    # Workloads are generated before the process touches the GPU (the
-    # batched one by forked workers).
+    # Workloads are generated before the process touches the GPU, the batched
+    # one by freshly started worker processes: a profiler that collects
+    # counters initialises the GPU before this script runs, and a process
+    # forked from one that has is not safe to run.
     _log(f"generating the 256-instance workload with {host_workers} processes")
-    wls3 = synth.config3(workers=host_workers)
+    wls3 = synth.config3(workers=host_workers, start_method="spawn")
     wl2 = synth.config2()
     wl5 = None
     if not args.no_fur:

@@ -774,9 +774,16 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_sdf_multi)(int phase, const tfx::SdfJo
             }
             break;
         case 1: {
-            constexpr int kTeam = 8;
-            const int tx = std::max(1, std::min((max_tris * kTeam + kConstructBlock - 1) / kConstructBlock, 4096));
-            hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kConstructBlock), 0, stream, jobs);
+            // 8 lanes per triangle (32 triangles per staged round) over many
+            // grids; a single grid takes 16 (smaller rounds, more blocks)
+            auto launch = [&](auto team) {
+                constexpr int kTeam = decltype(team)::value;
+                const int tx = std::max(1, std::min((max_tris * kTeam + kConstructBlock - 1) / kConstructBlock, 4096));
+                hipLaunchKernelGGL(k_sdf_construct_multi<kTeam>, dim3(tx, njobs), dim3(kConstructBlock), 0, stream,
+                                   jobs);
+            };
+            if (njobs == 1) launch(std::integral_constant<int, 16>{});
+            else launch(std::integral_constant<int, 8>{});
             break;
         }
         case 2:
