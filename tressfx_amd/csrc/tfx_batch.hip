@@ -41,6 +41,7 @@ struct tfx_batch {
     void* dev_slot = nullptr;          // device copy of a slot
     DevHair* dev_tables = nullptr;     // [cap][n]
     tfx::WorkItem* dev_items = nullptr;
+    uint32_t* dev_queue = nullptr;     // the fused launches' work queues (kFusedQueueWords)
     int vbegin[65] = {0}, vcount[65] = {0}, vvariant[65] = {0}, vresident[65] = {0};
     int vdefer[65] = {0};              // class V's fused launch leaves the SDF collision to the collide pass
     int cbegin = 0, ccount = 0;        // collide pass work items (member, first vertex) in dev_items
@@ -67,9 +68,11 @@ static void release_device(tfx_batch* b) {
     (void)hipFree(b->dev_slot);
     (void)hipFree(b->dev_tables);
     (void)hipFree(b->dev_items);
+    (void)hipFree(b->dev_queue);
     b->dev_slot = nullptr;
     b->dev_tables = nullptr;
     b->dev_items = nullptr;
+    b->dev_queue = nullptr;
     for (int k = 0; k < 2; ++k) {
         if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
         b->exec[k] = nullptr;
@@ -203,6 +206,9 @@ static int prepare(tfx_batch* b) {
     if (n > 0) {
         HIP_TRY(hipMalloc((void**)&b->dev_tables, tables.size() * sizeof(DevHair)));
         HIP_TRY(hipMemcpy(b->dev_tables, tables.data(), tables.size() * sizeof(DevHair), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc((void**)&b->dev_queue, tfx::kFusedQueueWords * sizeof(uint32_t)));
+        HIP_TRY(hipMemset(b->dev_queue, 0, tfx::kFusedQueueWords * sizeof(uint32_t)));
+        HIP_TRY(hipDeviceSynchronize());
     }
     int rc = build_work_lists(b);
     if (rc) return rc;
@@ -391,7 +397,8 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
                 const int grid = std::min(b->vcount[V], b->vresident[V]);
                 PassTimer tm(timers ? fused_label(V, kl_fused_block_vertices(V, b->vvariant[V])) : nullptr, s, true);
                 HIP_TRY(tm.launched(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V],
-                                                    b->vcount[V], grid, 1, s, tm.start(), tm.stop())));
+                                                    b->vcount[V], grid, 1, b->dev_queue, s, tm.start(),
+                                                    tm.stop())));
             }
             HIP_TRY(hipStreamWaitEvent(s, b->join, 0));
         }
@@ -401,7 +408,8 @@ static int launch_frame(tfx_batch* b, hipStream_t s, int substeps, int timers) {
             const int grid = std::min(b->vcount[V], b->vresident[V]);
             PassTimer tm(timers ? fused_label(V, kl_fused_block_vertices(V, b->vvariant[V])) : nullptr, s, true);
             HIP_TRY(tm.launched(kl_launch_fused(V, b->vvariant[V], tables, b->dev_items + b->vbegin[V],
-                                                b->vcount[V], grid, b->vdefer[V], s, tm.start(), tm.stop())));
+                                                b->vcount[V], grid, b->vdefer[V], b->dev_queue, s, tm.start(),
+                                                tm.stop())));
         }
         if (b->ccount) {
             PassTimer tm(timers ? "CollideHairVerticesWithSdf" : nullptr, s, true);
@@ -530,7 +538,7 @@ extern "C" int tfx_batch_autotune(tfx_batch* b, void* stream, int repeats, tfx_a
                             return hip_fail(e, "restore");
                     if ((e = hipEventRecord(e0, s)) != hipSuccess) return hip_fail(e, "event");
                     if ((e = kl_launch_fused(V, variant, b->dev_tables, items_d, (int)items.size(), grid, defer,
-                                             s, nullptr, nullptr)) != hipSuccess ||
+                                             b->dev_queue, s, nullptr, nullptr)) != hipSuccess ||
                         (defer && (e = kl_launch_collide_items(b->dev_tables, citems_d, (int)citems.size(), s,
                                                                nullptr, nullptr)) != hipSuccess))
                         return hip_fail(e, "launch");

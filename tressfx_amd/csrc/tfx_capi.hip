@@ -480,6 +480,7 @@ extern "C" int tfx_simulation_destroy(tfx_simulation* s) {
     if (!s) return TFX_OK;
     (void)hipDeviceSynchronize();
     (void)hipFree(s->table_d);
+    (void)hipFree(s->queue_d);
     delete s;
     return TFX_OK;
 }
@@ -536,6 +537,11 @@ extern "C" int tfx_simulation_set_launch_shape(tfx_simulation* sim, int V, int v
 }
 
 static int ensure_table(tfx_simulation* sim, size_t bytes) {
+    if (!sim->queue_d) {
+        HIP_TRY(hipMalloc((void**)&sim->queue_d, tfx::kFusedQueueWords * sizeof(uint32_t)));
+        HIP_TRY(hipMemset(sim->queue_d, 0, tfx::kFusedQueueWords * sizeof(uint32_t)));
+        HIP_TRY(hipDeviceSynchronize());
+    }
     if (bytes <= sim->table_bytes) return TFX_OK;
     if (sim->table_d) { HIP_TRY(hipDeviceSynchronize()); HIP_TRY(hipFree(sim->table_d)); sim->table_d = nullptr; }
     size_t cap = std::max<size_t>(bytes * 2, 1 << 16);
@@ -614,7 +620,7 @@ extern "C" int tfx_simulation_simulate_fused(tfx_simulation* sim, void* stream, 
         const int grid = std::min(cnt, vclass_resident[V]);
         PassTimer tm(fused_label(V, kl_fused_block_vertices(V, vclass_variant[V])), s, true);
         HIP_TRY(tm.launched(kl_launch_fused(V, vclass_variant[V], dtab, ditems + vclass_begin[V], cnt, grid,
-                                            defer[V], s, tm.start(), tm.stop())));
+                                            defer[V], sim->queue_d, s, tm.start(), tm.stop())));
     }
     if (ccount) {
         PassTimer tm("CollideHairVerticesWithSdf", s, true);
@@ -643,7 +649,8 @@ constexpr size_t kNaParamsOff = 0;
 constexpr size_t kNaHairsOff = na_align(kNaMaxHairs * sizeof(DevParams));
 constexpr size_t kNaBonesOff = kNaHairsOff + na_align(kNaMaxHairs * sizeof(DevHair));
 constexpr size_t kNaItemsOff = kNaBonesOff + na_align((size_t)kNaMaxBones * 16 * sizeof(float));
-constexpr size_t kNaScratchBytes = kNaItemsOff + (size_t)kNaMaxItems * sizeof(tfx::WorkItem);
+constexpr size_t kNaQueueOff = kNaItemsOff + na_align((size_t)kNaMaxItems * sizeof(tfx::WorkItem));
+constexpr size_t kNaScratchBytes = kNaQueueOff + tfx::kFusedQueueWords * sizeof(uint32_t);
 
 // One staging launch: the object's parameter block and table row (when
 // nitems > 0, its first launch), a chunk of its bone matrices, its work items.
@@ -668,6 +675,10 @@ __global__ void k_noalloc_stage(unsigned char* base, NaStage st) {
         params[st.slot] = st.params;
         hairs[st.slot] = st.hair;
     }
+    // the fused launches' work queues start at zero (every completed launch
+    // also leaves them there)
+    if (st.slot == 0 && st.bone_first == 0 && t < tfx::kFusedQueueWords)
+        reinterpret_cast<uint32_t*>(base + kNaQueueOff)[t] = 0u;
     for (int k = t; k < st.nbones * 16; k += stride) bones[st.bone_first * 16 + k] = st.bones[k];
     for (int k = t; k < st.nitems; k += stride) items[st.item_first + k] = tfx::WorkItem{st.slot, k * st.block_strands};
 }
@@ -774,7 +785,7 @@ extern "C" int tfx_simulation_simulate_fused_noalloc(tfx_simulation* sim, void* 
         const int cnt = cls_count[V];
         if (!cnt) continue;
         HIP_TRY(kl_launch_fused(V, cls_variant[V], dtab, ditems + cls_begin[V], cnt, std::min(cnt, cls_resident[V]), 0,
-                                s, nullptr, nullptr));
+                                reinterpret_cast<uint32_t*>(b + kNaQueueOff), s, nullptr, nullptr));
     }
     for (int i = 0; i < n; ++i) tfx_hair_increase_simulation_frame(hairs[i]);
     return TFX_OK;

@@ -499,6 +499,15 @@ struct WorkItem {
     int g0;     // first guide strand of the block
 };
 
+// Device words a caller of the fused launches keeps for their work queues:
+// two per strand-length class (V = 4, 8, 16, 32, 64), zero when a launch
+// starts, and left at zero by every launch that completes.  Launches that
+// share the words must not overlap in time (one stream does that).
+constexpr int kFusedQueueWords = 16;
+__host__ __device__ constexpr int fused_queue_class(int V) {
+    return V == 4 ? 0 : V == 8 ? 1 : V == 16 ? 2 : V == 32 ? 3 : 4;
+}
+
 // SDF grid constants (TressFXSDFCollisionParams subset).
 struct SdfGrid {
     float ox, oy, oz, cell;

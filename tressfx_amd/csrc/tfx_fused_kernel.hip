@@ -115,7 +115,7 @@ struct FusedShape {
 template <int V, int VB>
 __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_substep(const DevHair* __restrict__ hairs,
                                                                const WorkItem* __restrict__ items, int nitems,
-                                                               int defer_sdf) {
+                                                               int defer_sdf, uint32_t* __restrict__ queue) {
     using S = FusedShape<V, VB>;
     constexpr int B = S::B, NPT = S::NPT, STRIDE = S::STRIDE;
     __shared__ float4 sPos[B * STRIDE];
@@ -132,6 +132,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     __shared__ uint32_t sBits[kMaxBrickWords];    // SDF brick occupancy, one bit per brick
     __shared__ int sIdx[B];            // P1->P6b the item's guide strands (the object's processing order)
     __shared__ int sQueueCount;
+    __shared__ uint32_t sNext;         // the block's next work item
 
     const int t = threadIdx.x;
     // Lane-per-strand phases use threads tt = t rotated by a block-dependent
@@ -166,7 +167,16 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     // The occupancy sBits holds (null: none).
     const uint32_t* bits_in_lds = nullptr;
 
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    // Work items are taken from the launch's queue (queue[0]: the next item
+    // to hand out), one at a time as each block gets to its next item, so
+    // that blocks whose items cost more take fewer of them and the blocks of
+    // a launch finish together.  The block takes its next item at the start
+    // of P1 (the atomic's round trip overlaps the phase's loads) and reads it
+    // in P4, where the idle waves start on it.
+    if (t == 0) sNext = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    uint32_t w = sNext;
+    while (w < (uint32_t)nitems) {
         const WorkItem item = items[w];
         const int hi = __builtin_amdgcn_readfirstlane(item.hair);
         const int g0 = __builtin_amdgcn_readfirstlane(item.g0);
@@ -205,6 +215,8 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         // Every global load of the phase is issued before any of its results
         // is used (lanes past the block's last strand read the first strand's
         // vertices and discard them).
+        uint32_t taken = 0;
+        if (t == 0) taken = atomicAdd(&queue[0], 1u);
         float4 prevReg[NPT];
         float4 restIn[NPT], posIn[NPT], prevIn[NPT];
 #pragma unroll
@@ -292,6 +304,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 }
             }
         }
+        if (t == 0) sNext = taken;
         __syncthreads();
 
         // ---- P2: strand-level VSP transform --------------------------------
@@ -359,8 +372,8 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         // lanes of a group are idle).
         const int lanes = npass >= 3 ? kLscPipe : npass;
         const int lsc_threads = min(((nb * lanes + 63) >> 6) << 6, kFusedBlock);
-        const int wn = w + (int)gridDim.x;
-        const bool ahead = wn < nitems && lsc_threads < kFusedBlock;
+        const uint32_t wn = sNext;
+        const bool ahead = wn < (uint32_t)nitems && lsc_threads < kFusedBlock;
         if (ahead && tt >= lsc_threads) {
             const WorkItem nx = items[wn];
             const DevHair hn = load_uniform(hairs + __builtin_amdgcn_readfirstlane(nx.hair));
@@ -709,19 +722,27 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             }
         }
         __syncthreads();  // LDS is reused by the next work item
+        w = wn;
+    }
+    // The queue is left at zero for the next launch: each block counts
+    // itself out (queue[1]) after its last take, so the last one to do so
+    // knows that every take of the launch has been made.
+    if (t == 0 && atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+        __hip_atomic_store(&queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 template <int V, int VB>
 static hipError_t launch_fused(const DevHair* hairs, const WorkItem* items, int nitems, int grid, int defer_sdf,
-                               hipStream_t stream, hipEvent_t t0, hipEvent_t t1) {
+                               uint32_t* queue, hipStream_t stream, hipEvent_t t0, hipEvent_t t1) {
     if (nitems <= 0) return hipSuccess;
     if (t0)
         hipExtLaunchKernelGGL((k_fused_substep<V, VB>), dim3(grid), dim3(kFusedBlock), 0, stream, t0, t1, 0, hairs,
-                              items, nitems, defer_sdf);
+                              items, nitems, defer_sdf, queue);
     else
         hipLaunchKernelGGL((k_fused_substep<V, VB>), dim3(grid), dim3(kFusedBlock), 0, stream, hairs, items, nitems,
-                           defer_sdf);
+                           defer_sdf, queue);
     return hipGetLastError();
 }
 
@@ -788,23 +809,25 @@ extern "C" hipError_t TFX_KSYM(tfx_launch_collide_items)(const tfx::DevHair* hai
 extern "C" int TFX_KSYM(tfx_collide_chunk)() { return tfx::kCollideBlock; }
 
 extern "C" hipError_t TFX_KSYM(tfx_launch_fused)(int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items,
-                                       int nitems, int grid, int defer_sdf, hipStream_t stream,
-                                       hipEvent_t t0, hipEvent_t t1) {
+                                       int nitems, int grid, int defer_sdf, uint32_t* queue,
+                                       hipStream_t stream, hipEvent_t t0, hipEvent_t t1) {
     using namespace tfx;
     const int vb = TFX_KSYM(tfx_fused_block_vertices)(V, variant);
+    if (queue == nullptr) return hipErrorInvalidValue;
+    uint32_t* q = queue + 2 * fused_queue_class(V);
     switch (V * 10000 + vb) {
-        case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 8 * 10000 + 1024: return launch_fused<8, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 16 * 10000 + 512: return launch_fused<16, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 16 * 10000 + 1024: return launch_fused<16, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 16 * 10000 + 2048: return launch_fused<16, 2048>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 32 * 10000 + 512: return launch_fused<32, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 32 * 10000 + 1024: return launch_fused<32, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 32 * 10000 + 2048: return launch_fused<32, 2048>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 64 * 10000 + 512: return launch_fused<64, 512>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 64 * 10000 + 1024: return launch_fused<64, 1024>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
-        case 64 * 10000 + 2048: return launch_fused<64, 2048>(hairs, items, nitems, grid, defer_sdf, stream, t0, t1);
+        case 4 * 10000 + 512: return launch_fused<4, 512>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 8 * 10000 + 512: return launch_fused<8, 512>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 8 * 10000 + 1024: return launch_fused<8, 1024>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 16 * 10000 + 512: return launch_fused<16, 512>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 16 * 10000 + 1024: return launch_fused<16, 1024>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 16 * 10000 + 2048: return launch_fused<16, 2048>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 32 * 10000 + 512: return launch_fused<32, 512>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 32 * 10000 + 1024: return launch_fused<32, 1024>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 32 * 10000 + 2048: return launch_fused<32, 2048>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 64 * 10000 + 512: return launch_fused<64, 512>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 64 * 10000 + 1024: return launch_fused<64, 1024>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
+        case 64 * 10000 + 2048: return launch_fused<64, 2048>(hairs, items, nitems, grid, defer_sdf, q, stream, t0, t1);
         default: return hipErrorInvalidValue;
     }
 }

@@ -53,8 +53,8 @@ TFX_DUAL(hipError_t, launch_sdf_collide_mixed,
          (pos, prev, n, sdf, g, margin, stream))
 TFX_DUAL(hipError_t, launch_fused,
          (int V, int variant, const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, int grid,
-          int defer_sdf, hipStream_t stream, hipEvent_t t0, hipEvent_t t1),
-         (V, variant, hairs, items, nitems, grid, defer_sdf, stream, t0, t1))
+          int defer_sdf, uint32_t* queue, hipStream_t stream, hipEvent_t t0, hipEvent_t t1),
+         (V, variant, hairs, items, nitems, grid, defer_sdf, queue, stream, t0, t1))
 TFX_DUAL(hipError_t, launch_collide_items,
          (const tfx::DevHair* hairs, const tfx::WorkItem* items, int nitems, hipStream_t stream, hipEvent_t t0,
           hipEvent_t t1),
@@ -272,6 +272,7 @@ struct tfx_simulation {
     Staging staging;
     void* table_d = nullptr;   // DevHair[] followed by WorkItem[]
     size_t table_bytes = 0;
+    uint32_t* queue_d = nullptr;  // the fused launches' work queues (kFusedQueueWords)
     int variant[65];           // launch-shape variant per V (-1: chosen per call from occupancy)
     int per_cu[65];            // persistent blocks per CU per V (<= 0: the kernel's occupancy)
     int defer_sdf = TFX_SDF_DEFER_AUTO;  // batched form: SDF collision in the fused launch or a pass after it

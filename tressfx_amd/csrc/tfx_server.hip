@@ -431,7 +431,7 @@ int launch_pass(tfx_server* s, Pass& pass) {
             if (!cnt) continue;
             const int grid = std::min(cnt, shapes[V].resident);
             HIP_TRY(kl_launch_fused(V, shapes[V].variant, dtables + (size_t)sub * n, ditems + vbegin[V], cnt, grid,
-                                     defer[V], st, nullptr, nullptr));
+                                     defer[V], s->shape.queue_d, st, nullptr, nullptr));
         }
         if (ccount[sub])
             HIP_TRY(kl_launch_collide_items(dtables + (size_t)sub * n, ditems + cbegin, ccount[sub], st, nullptr,
@@ -533,7 +533,12 @@ extern "C" int tfx_server_create(const tfx_server_config* cfg, tfx_server** out)
     s->shape.device = cfg->device;
     for (int v = 0; v <= 64; ++v) s->shape.variant[v] = s->shape.per_cu[v] = -1;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->shape.queue_d, tfx::kFusedQueueWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(s->shape.queue_d, 0, tfx::kFusedQueueWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        (void)hipFree(s->shape.queue_d);
         delete s;
         return fail(TFX_ERR_HIP, "tfx_server_create: %s", hipGetErrorString(e));
     }
@@ -563,6 +568,7 @@ extern "C" int tfx_server_destroy(tfx_server* s) {
     }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(s->stream);
+    (void)hipFree(s->shape.queue_d);
     delete s;
     return TFX_OK;
 }
