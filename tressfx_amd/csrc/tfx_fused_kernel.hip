@@ -167,15 +167,17 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     // The occupancy sBits holds (null: none).
     const uint32_t* bits_in_lds = nullptr;
 
-    // Work items are taken from the launch's queue (queue[0]: the next item
-    // to hand out), one at a time as each block gets to its next item, so
-    // that blocks whose items cost more take fewer of them and the blocks of
-    // a launch finish together.  The block takes its next item at the start
-    // of P1 (the atomic's round trip overlaps the phase's loads) and reads it
-    // in P4, where the idle waves start on it.
-    if (t == 0) sNext = atomicAdd(&queue[0], 1u);
-    __syncthreads();
-    uint32_t w = sNext;
+    // Block b's first work item is item b; the items past the first round
+    // are taken from the launch's queue (queue[0] counts the takes, item
+    // gridDim.x + count), one at a time as each block gets to its next item,
+    // so that blocks whose items cost more take fewer of them and the blocks
+    // of a launch finish together.  A block takes its next item in P1, after
+    // the phase's stream loads (so that waiting for them does not wait for
+    // the atomic, which all blocks issue at once in their first item), and
+    // publishes it at the end of P3 for P4, where the idle waves start on it.
+    // A launch whose first round covers the list takes nothing.
+    const bool queued = (uint32_t)nitems > gridDim.x;
+    uint32_t w = blockIdx.x;
     while (w < (uint32_t)nitems) {
         const WorkItem item = items[w];
         const int hi = __builtin_amdgcn_readfirstlane(item.hair);
@@ -215,8 +217,6 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
         // Every global load of the phase is issued before any of its results
         // is used (lanes past the block's last strand read the first strand's
         // vertices and discard them).
-        uint32_t taken = 0;
-        if (t == 0) taken = atomicAdd(&queue[0], 1u);
         float4 prevReg[NPT];
         float4 restIn[NPT], posIn[NPT], prevIn[NPT];
 #pragma unroll
@@ -245,6 +245,8 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
             const size_t i = ((size_t)sIdx[ls < nb ? ls : 0] * (F + 1)) * V + v;
             restLen[m] = ldg(reinterpret_cast<const float2*>(&h.restlen[i]));
         }
+        uint32_t taken = (uint32_t)nitems;
+        if (queued && t == 0) taken = gridDim.x + atomicAdd(&queue[0], 1u);
         const bool reset = P.reset != 0.0f;
         const float grav = P.gravity, dt = P.dt, decay = P.decay;
         const bool gshape = P.global_stiffness > 0.0f && P.global_range != 0.0f;
@@ -304,7 +306,6 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 }
             }
         }
-        if (t == 0) sNext = taken;
         __syncthreads();
 
         // ---- P2: strand-level VSP transform --------------------------------
@@ -344,6 +345,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
                 prevReg[k].x = ro.x; prevReg[k].y = ro.y; prevReg[k].z = ro.z;
             }
         }
+        if (t == 0) sNext = taken;
         __syncthreads();
 
         // ---- P4: local shape constraints (Gauss–Seidel sweeps) -------------
@@ -727,7 +729,7 @@ __global__ void __launch_bounds__(kFusedBlock, (VB <= 512 ? 4 : 3)) k_fused_subs
     // The queue is left at zero for the next launch: each block counts
     // itself out (queue[1]) after its last take, so the last one to do so
     // knows that every take of the launch has been made.
-    if (t == 0 && atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+    if (queued && t == 0 && atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
         __hip_atomic_store(&queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
