@@ -534,6 +534,9 @@ def main():
     ap.add_argument("--no-fur", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--host-workers", type=int, default=0,
+                    help="processes generating the 256-instance workload (0: the host's CPU share, at most 16; "
+                         "1: in this process, for runs under a profiler whose preload every worker would load)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the legs, copy a device buffer of this many bytes 3 times (a known-size "
                          "kernel that calibrates HBM byte counters)")
@@ -546,7 +549,7 @@ def main():
     import numpy as np
     from tressfx_amd import synth
 
-    host_workers = max(1, min(16, cpu_threads()[0]))
+    host_workers = args.host_workers if args.host_workers > 0 else max(1, min(16, cpu_threads()[0]))
     # Workloads are generated before the process touches the GPU, the batched
     # one by freshly started worker processes: a profiler that collects
     # counters initialises the GPU before this script runs, and a process
